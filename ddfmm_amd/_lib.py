@@ -1,0 +1,78 @@
+"""ctypes binding of the C ABI in include/ddfmm.h (libddfmm.so, built in-tree)."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libddfmm.so")
+
+EXPORTS = ["ddfmm_last_error", "ddfmm_create", "ddfmm_destroy", "ddfmm_setup", "ddfmm_eval",
+           "ddfmm_eval_device", "ddfmm_direct_sum", "ddfmm_kernel_matrix", "ddfmm_get_stats",
+           "ddfmm_dump_lists", "ddfmm_set_timing", "ddfmm_stage_times"]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("K", ctypes.c_double), ("ctr", ctypes.c_double * 3), ("ACCU", ctypes.c_int),
+                ("NPQ", ctypes.c_int), ("ptsmax", ctypes.c_int), ("maxlevel", ctypes.c_int),
+                ("numC", ctypes.c_int), ("kernel", ctypes.c_int), ("nrhs_max", ctypes.c_int),
+                ("precision", ctypes.c_int)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in (
+        "npts", "nleaf", "nlf", "nhf_out", "nhf_in", "near_interactions", "leaf_op_interactions",
+        "s2m_interactions", "x2chk_interactions", "v_pairs", "v_targets", "v_sources", "m2l_pairs",
+        "m2l_groups")] + [("m2l_flops", ctypes.c_double), ("gemm_flops", ctypes.c_double),
+                          ("opmat_elems", ctypes.c_int64), ("device_bytes", ctypes.c_int64),
+                          ("nlevels_lf", ctypes.c_int), ("nlevels_hf", ctypes.c_int),
+                          ("max_hf_width", ctypes.c_int), ("nstages", ctypes.c_int)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class DdfmmError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libddfmm.so; raises if the HIP library has not been built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise DdfmmError(f"{LIB_PATH} is missing: build it with ddfmm_amd.build.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i64, ci, cd = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
+    L.ddfmm_last_error.restype = ctypes.c_char_p
+    L.ddfmm_create.argtypes = [ci, ctypes.POINTER(vp)]
+    L.ddfmm_destroy.argtypes = [vp]
+    L.ddfmm_setup.argtypes = [vp, vp, i64, ctypes.POINTER(Params), ctypes.c_char_p, ctypes.c_char_p]
+    L.ddfmm_eval.argtypes = [vp, vp, vp, ci]
+    L.ddfmm_eval_device.argtypes = [vp, vp, vp, ci, vp]
+    L.ddfmm_direct_sum.argtypes = [ci, vp, ci, vp, ci, vp, ci, vp]
+    L.ddfmm_kernel_matrix.argtypes = [ci, vp, ci, vp, ci, vp]
+    L.ddfmm_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+    L.ddfmm_dump_lists.argtypes = [vp, ctypes.c_char_p, i64, ctypes.POINTER(i64)]
+    L.ddfmm_set_timing.argtypes = [vp, ci]
+    L.ddfmm_stage_times.argtypes = [vp, ci, ctypes.c_char_p, ctypes.POINTER(cd), ctypes.POINTER(ci)]
+    for name in EXPORTS:
+        if name != "ddfmm_last_error":
+            getattr(L, name).restype = ci
+    _lib = L
+    return L
+
+
+def check(rc: int):
+    if rc != 0:
+        raise DdfmmError(lib().ddfmm_last_error().decode(errors="replace"))
+
+
+def ptr(a: np.ndarray) -> int:
+    return a.ctypes.data

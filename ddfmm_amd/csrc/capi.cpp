@@ -1,0 +1,353 @@
+// C ABI (include/ddfmm.h): context lifetime, setup (host planning + device upload +
+// translation-matrix generation) and evaluation (stage sequence on one HIP stream).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/ddfmm.h"
+#include "device.hpp"
+#include "planner.hpp"
+
+using namespace ddfmm;
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+struct HipFail : std::runtime_error {
+    explicit HipFail(const std::string& s) : std::runtime_error(s) {}
+};
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            throw HipFail(std::string(#expr) + ": " + hipGetErrorString(e_));                 \
+    } while (0)
+}  // namespace
+
+struct ddfmm_ctx {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    Params prm;
+    int nrhs_max = 1, precision = 0;
+    HostPlan hp;
+    DevPlan dp;
+    std::vector<void*> allocs;
+    int64_t device_bytes = 0;
+    bool ready = false;
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    std::vector<double> stage_ms;
+
+    template <class T>
+    T* alloc(int64_t n) {
+        if (n <= 0) return nullptr;
+        void* p = nullptr;
+        HIPCHK(hipMalloc(&p, (size_t)n * sizeof(T)));
+        allocs.push_back(p);
+        device_bytes += n * (int64_t)sizeof(T);
+        return (T*)p;
+    }
+    template <class T>
+    T* upload(const std::vector<T>& v) {
+        T* p = alloc<T>((int64_t)v.size());
+        if (p) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        return p;
+    }
+    void release() {
+        for (void* p : allocs) (void)hipFree(p);
+        allocs.clear();
+        for (auto e : ev) (void)hipEventDestroy(e);
+        ev.clear();
+        device_bytes = 0;
+        ready = false;
+    }
+};
+
+extern "C" {
+
+const char* ddfmm_last_error(void) { return g_err.c_str(); }
+
+int ddfmm_create(int device, ddfmm_ctx** out) {
+    if (!out) return fail(DDFMM_EINVAL, "null output pointer");
+    auto* c = new ddfmm_ctx();
+    c->device = device;
+    if (device >= 0) {
+        try {
+            HIPCHK(hipSetDevice(device));
+            HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        } catch (const std::exception& e) {
+            delete c;
+            return fail(DDFMM_EDEVICE, e.what());
+        }
+    }
+    *out = c;
+    return DDFMM_OK;
+}
+
+int ddfmm_destroy(ddfmm_ctx* c) {
+    if (!c) return DDFMM_OK;
+    if (c->device >= 0) {
+        (void)hipSetDevice(c->device);
+        if (c->stream) (void)hipStreamSynchronize(c->stream);
+        c->release();
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+    }
+    delete c;
+    return DDFMM_OK;
+}
+
+int ddfmm_setup(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_params* p,
+                const char* ldfile, const char* hdfile) {
+    if (!c || !pos || !p || !ldfile || !hdfile) return fail(DDFMM_EINVAL, "null argument");
+    if (n <= 0 || n > (int64_t)INT32_MAX) return fail(DDFMM_EINVAL, "point count out of range");
+    if (p->kernel != 0 && p->kernel != 1) return fail(DDFMM_EINVAL, "Unknown kernel type " + std::to_string(p->kernel));
+    if (p->ACCU < 1 || p->ACCU > 3) return fail(DDFMM_EINVAL, "ACCU must be 1, 2 or 3");
+    if (p->ACCU != 2) return fail(DDFMM_EINVAL, "this build evaluates ACCU=2 (P=6) only");
+    if (p->nrhs_max < 1) return fail(DDFMM_EINVAL, "nrhs_max must be >= 1");
+    if (p->precision != DDFMM_FP64) return fail(DDFMM_EINVAL, "unsupported precision mode");
+    double lk = std::log2(p->K);
+    if (!(p->K >= 1) || std::abs(lk - std::round(lk)) > 1e-12) return fail(DDFMM_EINVAL, "K must be a power of two >= 1");
+    try {
+        if (c->device >= 0) {
+            HIPCHK(hipSetDevice(c->device));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+        c->release();
+        c->hp = HostPlan();
+        Params& q = c->prm;
+        q.K = p->K;
+        for (int d = 0; d < 3; ++d) q.ctr[d] = p->ctr[d];
+        q.ACCU = p->ACCU; q.NPQ = p->NPQ; q.ptsmax = p->ptsmax; q.maxlevel = p->maxlevel;
+        q.numC = p->numC; q.ktype = p->kernel;
+        c->nrhs_max = p->nrhs_max;
+        c->precision = p->precision;
+        Mlib mlib;
+        mlib.load(ldfile, hdfile);
+        build_plan(q, pos, n, mlib, c->hp, n <= 2000000);
+    } catch (const HipFail& e) {
+        return fail(DDFMM_EDEVICE, e.what());
+    } catch (const std::exception& e) {
+        return fail(DDFMM_ESETUP, e.what());
+    }
+    if (c->device < 0) {
+        c->ready = true;
+        return DDFMM_OK;
+    }
+    try {
+        HostPlan& h = c->hp;
+        DevPlan& d = c->dp;
+        d = DevPlan();
+        d.N = h.N; d.P = h.P; d.NL = h.NL; d.ktype = c->prm.ktype;
+        d.px = c->upload(h.px); d.py = c->upload(h.py); d.pz = c->upload(h.pz);
+        d.perm = c->upload(h.perm);
+        d.grid = c->upload(h.grid);
+        d.opmat = c->alloc<double2>(h.opmat_total);
+        if (!h.opmat.empty())
+            HIPCHK(hipMemcpy(d.opmat, h.opmat.data(), h.opmat.size() * sizeof(double2), hipMemcpyHostToDevice));
+        d.ktasks = c->upload(h.ktasks); d.ksrcs = c->upload(h.ksrcs);
+        d.groups = c->upload(h.groups); d.tiles = c->upload(h.tiles);
+        d.colB = c->upload(h.colB); d.colC = c->upload(h.colC);
+        d.fsrc = c->upload(h.fsrc); d.vtrg = c->upload(h.vtrg); d.vpair = c->upload(h.vpair);
+        for (int b = 0; b < BUF_COUNT; ++b) {
+            d.bt.stride[b] = h.bufsize[b];
+            d.bt.p[b] = c->alloc<double2>(h.bufsize[b] * c->nrhs_max);
+        }
+        d.den_io = c->alloc<double2>(h.N * c->nrhs_max);
+        d.val_io = c->alloc<double2>(h.N * c->nrhs_max);
+        // translation matrices generated on the device (temporary generator tables)
+        if (!h.kgen.empty()) {
+            double* gp = nullptr;
+            KGen* jobs = nullptr;
+            HIPCHK(hipMalloc(&gp, h.genpts.size() * sizeof(double)));
+            HIPCHK(hipMalloc(&jobs, h.kgen.size() * sizeof(KGen)));
+            HIPCHK(hipMemcpy(gp, h.genpts.data(), h.genpts.size() * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(jobs, h.kgen.data(), h.kgen.size() * sizeof(KGen), hipMemcpyHostToDevice));
+            HIPCHK(launch_kgen(jobs, (int)h.kgen.size(), gp, d.opmat, d.ktype, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            HIPCHK(hipFree(gp));
+            HIPCHK(hipFree(jobs));
+        }
+        // host copies no longer needed on the device path
+        std::vector<cpx>().swap(h.opmat);
+        std::vector<double>().swap(h.genpts);
+        std::vector<KGen>().swap(h.kgen);
+        c->ready = true;
+    } catch (const HipFail& e) {
+        c->release();
+        return fail(DDFMM_EDEVICE, e.what());
+    } catch (const std::exception& e) {
+        c->release();
+        return fail(DDFMM_ENOMEM, e.what());
+    }
+    return DDFMM_OK;
+}
+
+static int run_stages(ddfmm_ctx* c, int nrhs, hipStream_t st) {
+    const auto& stages = c->hp.stages;
+    if (c->timing) {
+        if (c->ev.size() != stages.size() + 1) {
+            for (auto e : c->ev) (void)hipEventDestroy(e);
+            c->ev.assign(stages.size() + 1, nullptr);
+            for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
+        }
+        HIPCHK(hipEventRecord(c->ev[0], st));
+    }
+    for (size_t i = 0; i < stages.size(); ++i) {
+        hipError_t e = launch_stage(c->dp, stages[i], nrhs, st);
+        if (e != hipSuccess)
+            throw HipFail(std::string("stage ") + stages[i].name + ": " + hipGetErrorString(e));
+        if (c->timing) HIPCHK(hipEventRecord(c->ev[i + 1], st));
+    }
+    if (c->timing) {
+        HIPCHK(hipEventSynchronize(c->ev.back()));
+        c->stage_ms.assign(stages.size(), 0.0);
+        for (size_t i = 0; i < stages.size(); ++i) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
+            c->stage_ms[i] = ms;
+        }
+    }
+    return DDFMM_OK;
+}
+
+int ddfmm_eval(ddfmm_ctx* c, const double* den, double* val, int nrhs) {
+    if (!c || !den || !val) return fail(DDFMM_EINVAL, "null argument");
+    if (!c->ready || c->device < 0) return fail(DDFMM_ESTATE, "context not set up on a device");
+    if (nrhs < 1 || nrhs > c->nrhs_max) return fail(DDFMM_EINVAL, "nrhs out of range [1, nrhs_max]");
+    try {
+        HIPCHK(hipSetDevice(c->device));
+        size_t bytes = (size_t)c->hp.N * nrhs * sizeof(double2);
+        HIPCHK(hipMemcpyAsync(c->dp.den_io, den, bytes, hipMemcpyHostToDevice, c->stream));
+        run_stages(c, nrhs, c->stream);
+        HIPCHK(hipMemcpyAsync(val, c->dp.val_io, bytes, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } catch (const std::exception& e) {
+        return fail(DDFMM_EDEVICE, e.what());
+    }
+    return DDFMM_OK;
+}
+
+int ddfmm_eval_device(ddfmm_ctx* c, const void* den_dev, void* val_dev, int nrhs, void* stream) {
+    if (!c || !den_dev || !val_dev) return fail(DDFMM_EINVAL, "null argument");
+    if (!c->ready || c->device < 0) return fail(DDFMM_ESTATE, "context not set up on a device");
+    if (nrhs < 1 || nrhs > c->nrhs_max) return fail(DDFMM_EINVAL, "nrhs out of range [1, nrhs_max]");
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    double2* sd = c->dp.den_io;
+    double2* sv = c->dp.val_io;
+    c->dp.den_io = (double2*)den_dev;
+    c->dp.val_io = (double2*)val_dev;
+    int rc = DDFMM_OK;
+    try {
+        HIPCHK(hipSetDevice(c->device));
+        run_stages(c, nrhs, st);
+    } catch (const std::exception& e) {
+        rc = fail(DDFMM_EDEVICE, e.what());
+    }
+    c->dp.den_io = sd;
+    c->dp.val_io = sv;
+    return rc;
+}
+
+int ddfmm_direct_sum(int kernel, const double* trg, int m, const double* src, int n, const double* den, int nrhs,
+                     double* out) {
+    if (!trg || !src || !den || !out || m < 0 || n < 0 || nrhs < 1) return fail(DDFMM_EINVAL, "bad argument");
+    if (kernel != 0 && kernel != 1) return fail(DDFMM_EINVAL, "Unknown kernel type " + std::to_string(kernel));
+    if (m == 0) return DDFMM_OK;
+    double *dt = nullptr, *ds = nullptr;
+    double2 *dd = nullptr, *dout = nullptr;
+    int rc = DDFMM_OK;
+    try {
+        HIPCHK(hipMalloc(&dt, (size_t)m * 3 * sizeof(double)));
+        HIPCHK(hipMalloc(&ds, (size_t)std::max(n, 1) * 3 * sizeof(double)));
+        HIPCHK(hipMalloc(&dd, (size_t)std::max(n, 1) * nrhs * sizeof(double2)));
+        HIPCHK(hipMalloc(&dout, (size_t)m * nrhs * sizeof(double2)));
+        HIPCHK(hipMemcpy(dt, trg, (size_t)m * 3 * sizeof(double), hipMemcpyHostToDevice));
+        if (n > 0) {
+            HIPCHK(hipMemcpy(ds, src, (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(dd, den, (size_t)n * nrhs * sizeof(double2), hipMemcpyHostToDevice));
+        }
+        HIPCHK(launch_direct(dt, m, ds, n, dd, nrhs, dout, kernel, nullptr));
+        HIPCHK(hipMemcpy(out, dout, (size_t)m * nrhs * sizeof(double2), hipMemcpyDeviceToHost));
+    } catch (const std::exception& e) {
+        rc = fail(DDFMM_EDEVICE, e.what());
+    }
+    (void)hipFree(dt); (void)hipFree(ds); (void)hipFree(dd); (void)hipFree(dout);
+    return rc;
+}
+
+int ddfmm_kernel_matrix(int kernel, const double* trg, int m, const double* src, int n, double* out) {
+    if (!trg || !src || !out || m < 0 || n < 0) return fail(DDFMM_EINVAL, "bad argument");
+    if (kernel != 0 && kernel != 1) return fail(DDFMM_EINVAL, "Unknown kernel type " + std::to_string(kernel));
+    if ((int64_t)m * n == 0) return DDFMM_OK;
+    double *dt = nullptr, *ds = nullptr;
+    double2* dout = nullptr;
+    int rc = DDFMM_OK;
+    try {
+        HIPCHK(hipMalloc(&dt, (size_t)m * 3 * sizeof(double)));
+        HIPCHK(hipMalloc(&ds, (size_t)n * 3 * sizeof(double)));
+        HIPCHK(hipMalloc(&dout, (size_t)m * n * sizeof(double2)));
+        HIPCHK(hipMemcpy(dt, trg, (size_t)m * 3 * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ds, src, (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(launch_kmatrix(dt, m, ds, n, dout, kernel, nullptr));
+        HIPCHK(hipMemcpy(out, dout, (size_t)m * n * sizeof(double2), hipMemcpyDeviceToHost));
+    } catch (const std::exception& e) {
+        rc = fail(DDFMM_EDEVICE, e.what());
+    }
+    (void)hipFree(dt); (void)hipFree(ds); (void)hipFree(dout);
+    return rc;
+}
+
+int ddfmm_get_stats(ddfmm_ctx* c, ddfmm_stats* s) {
+    if (!c || !s) return fail(DDFMM_EINVAL, "null argument");
+    if (!c->ready) return fail(DDFMM_ESTATE, "context not set up");
+    const PlanStats& p = c->hp.stats;
+    std::memset(s, 0, sizeof(*s));
+    s->npts = p.npts; s->nleaf = p.nleaf; s->nlf = p.nlf; s->nhf_out = p.nhf_out; s->nhf_in = p.nhf_in;
+    s->near_interactions = p.near_interactions; s->leaf_op_interactions = p.leaf_op_interactions;
+    s->s2m_interactions = p.s2m_interactions; s->x2chk_interactions = p.x2chk_interactions;
+    s->v_pairs = p.v_pairs; s->v_targets = p.v_targets; s->v_sources = p.v_sources;
+    s->m2l_pairs = p.m2l_pairs; s->m2l_groups = p.m2l_groups;
+    s->m2l_flops = p.m2l_flops; s->gemm_flops = p.gemm_flops;
+    s->opmat_elems = p.opmat_elems; s->device_bytes = c->device_bytes;
+    s->nlevels_lf = p.nlevels_lf; s->nlevels_hf = p.nlevels_hf; s->max_hf_width = p.max_hf_width;
+    s->nstages = (int)c->hp.stages.size();
+    return DDFMM_OK;
+}
+
+int ddfmm_dump_lists(ddfmm_ctx* c, char* buf, int64_t cap, int64_t* len) {
+    if (!c || !len) return fail(DDFMM_EINVAL, "null argument");
+    if (!c->ready) return fail(DDFMM_ESTATE, "context not set up");
+    *len = (int64_t)c->hp.dump.size();
+    if (buf && cap > 0) std::memcpy(buf, c->hp.dump.data(), (size_t)std::min<int64_t>(cap, *len));
+    return DDFMM_OK;
+}
+
+int ddfmm_set_timing(ddfmm_ctx* c, int on) {
+    if (!c) return fail(DDFMM_EINVAL, "null argument");
+    c->timing = on != 0;
+    return DDFMM_OK;
+}
+
+int ddfmm_stage_times(ddfmm_ctx* c, int cap, char* names, double* ms, int* count) {
+    if (!c || !count) return fail(DDFMM_EINVAL, "null argument");
+    const auto& st = c->hp.stages;
+    *count = (int)st.size();
+    for (int i = 0; i < (int)st.size() && i < cap; ++i) {
+        if (names) std::memcpy(names + 24 * i, st[i].name, 24);
+        if (ms) ms[i] = i < (int)c->stage_ms.size() ? c->stage_ms[i] : 0.0;
+    }
+    return DDFMM_OK;
+}
+
+}  // extern "C"

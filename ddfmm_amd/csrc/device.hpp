@@ -1,0 +1,42 @@
+// Device-side view of an evaluation plan and the kernel launchers (kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "plan.hpp"
+
+namespace ddfmm {
+
+struct BufTable {
+    double2* p[BUF_COUNT];
+    int64_t stride[BUF_COUNT];     // elements per rhs slab
+};
+
+struct DevPlan {
+    int64_t N = 0;
+    int P = 0, NL = 0, ktype = 0;
+    double *px = nullptr, *py = nullptr, *pz = nullptr;
+    int* perm = nullptr;
+    int* grid = nullptr;
+    double2* opmat = nullptr;
+    KTask* ktasks = nullptr;
+    KSrc* ksrcs = nullptr;
+    GGroup* groups = nullptr;
+    GTile* tiles = nullptr;
+    int64_t *colB = nullptr, *colC = nullptr;
+    FftSrc* fsrc = nullptr;
+    VTarget* vtrg = nullptr;
+    VPair* vpair = nullptr;
+    BufTable bt{};
+    double2* den_io = nullptr;     // [rhs][N] input order (gather source)
+    double2* val_io = nullptr;     // [rhs][N] input order (scatter target)
+};
+
+hipError_t launch_stage(const DevPlan& d, const Stage& s, int nrhs, hipStream_t st);
+hipError_t launch_kgen(const KGen* jobs, int njobs, const double* gp, double2* opmat, int ktype, hipStream_t st);
+hipError_t launch_direct(const double* trg, int M, const double* src, int Ns, const double2* den, int nrhs,
+                         double2* out, int ktype, hipStream_t st);
+hipError_t launch_kmatrix(const double* trg, int M, const double* src, int Ns, double2* out, int ktype,
+                          hipStream_t st);
+
+}  // namespace ddfmm

@@ -1,0 +1,105 @@
+// Flattened evaluation plan: the octree, lists and operators of a Wave3d setup turned
+// into device-resident descriptor tables.  The evaluation (wave3d_eval.cpp:299-415) is
+// a fixed sequence of stages; every stage is one kernel launch over a contiguous range
+// of descriptors.  Shared by the host planner and the HIP kernels, so POD only.
+#pragma once
+
+#include <cstdint>
+
+namespace ddfmm {
+
+// vector buffers ([rhs][...] complex, one rhs slab per right-hand side)
+enum Buf : int {
+    BUF_DEN = 0,    // leaf-sorted densities, N
+    BUF_VAL,        // leaf-sorted potentials, N
+    BUF_LF_UPEQ,    // LF upward equivalent densities, nLF x NL
+    BUF_LF_CHK,     // LF upward check values, then downward check values
+    BUF_LF_TMP,     // LF pinv middle vector
+    BUF_LF_DNEQ,    // LF downward equivalent densities
+    BUF_HF_UP,      // HF directional upward equivalent densities (out-slots)
+    BUF_HF_DN,      // HF directional downward check values (in-slots)
+    BUF_HF_A,       // HF per-level scratch
+    BUF_HF_B,
+    BUF_SPEC,       // LF V-list source spectra of one level, (2P)^3 each
+    BUF_COUNT
+};
+
+// kernel-apply: out(target_i) (+)= sum_src sum_j G(t_i, s_j) v_j
+// point coordinates: pts[off + i] * scale + c
+struct KSrc {
+    int64_t off;
+    int64_t voff;
+    double c[3];
+    double scale;
+    int count;
+    int vbuf;
+};
+struct KTask {
+    int64_t off;
+    int64_t ooff;
+    double c[3];
+    double scale;
+    int count;
+    int obuf;
+    int s0, s1;
+    int mode;       // 0 store, 1 add
+    int pad;
+};
+
+// grouped complex GEMM: C[:, ccol] (=, +=, atomic +=) diag(scale) * op(A) * B[:, bcol]
+struct GGroup {
+    int64_t aoff;   // A offset in the operator table
+    int64_t soff;   // row scale (length M) offset, or -1
+    int M, K, lda, transA;
+    int bbuf, cbuf;
+    int ncols, col0;
+    int mode;       // 0 store, 1 add, 2 atomic add
+    int pad;
+};
+struct GTile {
+    int g, mt, nt, pad;
+};
+constexpr int GEMM_TM = 64;
+constexpr int GEMM_TN = 32;
+
+// LF FFT-diagonal M2L (wave3d_eval.cpp:602-661)
+struct FftSrc {
+    int64_t in_off;     // LF_UPEQ slot
+    int64_t spec_off;   // SPEC slot
+    int grid;           // grid-index table offset
+    int pad;
+};
+struct VTarget {
+    int64_t out_off;    // LF_CHK slot
+    int p0, p1;         // pair range
+    int64_t tab_off;    // translation tensor table of this width (343 x (2P)^3)
+    int grid;
+    int pad;
+};
+struct VPair {
+    int64_t spec_off;
+    int offidx;         // (a+3) + 7(b+3) + 49(c+3)
+    int pad;
+};
+
+// setup-time kernel matrix generation: out(i,j) = G((sgn*T_i + tshift) - (S_j + sshift))
+struct KGen {
+    int64_t toff, soff, out;   // offsets into the generator point table / operator table
+    double tshift[3];
+    double sshift[3];
+    int m, n;
+    int tneg, pad;
+};
+
+enum StageKind : int {
+    ST_ZERO = 0, ST_GATHER, ST_SCATTER, ST_KAPPLY, ST_GEMM, ST_FFTFWD, ST_VLIST
+};
+struct Stage {
+    int kind;
+    int begin, end;     // descriptor range (tasks / tiles / sources / targets)
+    int buf;            // ST_ZERO: buffer id
+    int64_t count;      // ST_ZERO: elements per rhs
+    char name[24];
+};
+
+}  // namespace ddfmm

@@ -1,0 +1,54 @@
+// Host planner: builds the tree (Tree), fetches operators (Mlib) and flattens every
+// stage of Wave3d::eval into descriptor tables (HostPlan) for the device.
+#pragma once
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "host.hpp"
+#include "plan.hpp"
+
+namespace ddfmm {
+
+struct PlanStats {
+    int64_t npts = 0, nleaf = 0, nlf = 0, nhf_out = 0, nhf_in = 0;
+    int64_t near_interactions = 0;      // sum over U/W/X-direct pairs of n_t * n_s (leaf kernel, point sources)
+    int64_t leaf_op_interactions = 0;   // leaf kernel interactions with operator point sets (L2T, W via upeq)
+    int64_t s2m_interactions = 0, x2chk_interactions = 0;
+    int64_t v_pairs = 0, v_targets = 0, v_sources = 0;
+    int64_t m2l_pairs = 0, m2l_groups = 0;
+    double m2l_flops = 0, gemm_flops = 0;   // useful complex-GEMM flops (8*M*K*N) per RHS
+    int64_t opmat_elems = 0, opmat_generated = 0;
+    int nlevels_lf = 0, nlevels_hf = 0, max_hf_width = 0;
+};
+
+struct HostPlan {
+    Params prm;
+    int64_t N = 0;
+    int P = 0, NL = 0;
+    std::vector<int> perm;                 // sorted position -> input index
+    std::vector<double> px, py, pz;        // leaf-sorted points then LF operator point sets
+    std::vector<cpx> opmat;                // uploaded operator tables (prefix of the table)
+    int64_t opmat_total = 0;               // uploaded + generated
+    std::vector<int> grid;                 // FFT grid index tables
+    std::vector<double> genpts;            // kernel-generation point table (xyz rows)
+    std::vector<KGen> kgen;
+    std::vector<KTask> ktasks;
+    std::vector<KSrc> ksrcs;
+    std::vector<GGroup> groups;
+    std::vector<GTile> tiles;
+    std::vector<int64_t> colB, colC;
+    std::vector<FftSrc> fsrc;
+    std::vector<VTarget> vtrg;
+    std::vector<VPair> vpair;
+    std::vector<Stage> stages;
+    int64_t bufsize[BUF_COUNT] = {0};      // elements per rhs
+    PlanStats stats;
+    std::string dump;                      // canonical text of the lists (debug/parity)
+};
+
+void build_plan(const Params& prm, const double* pos, int64_t N, const Mlib& mlib,
+                HostPlan& plan, bool want_dump);
+
+}  // namespace ddfmm

@@ -1,0 +1,91 @@
+/* C ABI of the MI355X directional-FMM evaluation pass.
+ *
+ * The reference exposes this path as C++ classes (no FFI): Wave3d setup/eval/check
+ * (include/wave3d.hpp:219-265), Kernel3d::kernel (include/kernel3d.hpp:29-45) and
+ * Mlib3d::setup (include/mlib3d.hpp:58-99).  Each entry point below states which
+ * of those it replaces.  All functions return 0 on success and a nonzero code on
+ * failure; ddfmm_last_error() then describes the failure (no exception crosses
+ * the ABI).  Complex arrays are interleaved (re, im) doubles.
+ */
+#ifndef DDFMM_H
+#define DDFMM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ddfmm_ctx ddfmm_ctx;
+
+enum { DDFMM_OK = 0, DDFMM_EINVAL = 1, DDFMM_ESETUP = 2, DDFMM_EDEVICE = 3, DDFMM_ESTATE = 4,
+       DDFMM_ENOMEM = 5 };
+enum { DDFMM_KERNEL_HELM = 0, DDFMM_KERNEL_EXPR = 1 };   /* kernel3d.hpp:24-27 */
+enum { DDFMM_FP64 = 0, DDFMM_MIXED = 1 };
+
+/* Wave3d / Mlib3d / Kernel3d options (wave3d_setup.cpp:29-61, mlib3d.cpp:70-91,
+   tt.cpp:103-119): K, ctr, ACCU, NPQ, ptsmax, maxlevel, cell count per edge numC
+   (geomprtn.m(), wave3d.hpp:305), kernel type.  nrhs_max sizes the device buffers. */
+typedef struct ddfmm_params {
+    double K;
+    double ctr[3];
+    int ACCU, NPQ, ptsmax, maxlevel, numC, kernel;
+    int nrhs_max;
+    int precision;
+} ddfmm_params;
+
+typedef struct ddfmm_stats {
+    int64_t npts, nleaf, nlf, nhf_out, nhf_in;
+    int64_t near_interactions, leaf_op_interactions, s2m_interactions, x2chk_interactions;
+    int64_t v_pairs, v_targets, v_sources, m2l_pairs, m2l_groups;
+    double m2l_flops, gemm_flops;
+    int64_t opmat_elems, device_bytes;
+    int nlevels_lf, nlevels_hf, max_hf_width, nstages;
+} ddfmm_stats;
+
+const char* ddfmm_last_error(void);
+
+/* Create a context on HIP device `device`; device < 0 gives a host-only context that
+   plans but never touches a GPU (tree/list inspection).  Replaces Wave3d(prefix)
+   (wave3d.cpp:24-31). */
+int ddfmm_create(int device, ddfmm_ctx** out);
+int ddfmm_destroy(ddfmm_ctx* ctx);
+
+/* Build the octree, interaction lists and direction sets on the host, read the
+   operator library (ldfile/hdfile in the reference binary format), generate the
+   translation matrices on the device and keep everything resident.  Replaces
+   Mlib3d::setup (mlib3d.cpp:61-103) + Wave3d::setup (wave3d_setup.cpp:21-97).
+   pos: n x 3 row-major doubles (point id = row). */
+int ddfmm_setup(ddfmm_ctx* ctx, const double* pos, int64_t n, const ddfmm_params* prm,
+                const char* ldfile, const char* hdfile);
+
+/* Potentials val = G * den for nrhs density vectors (host buffers, nrhs x n complex,
+   point-id order).  Replaces Wave3d::eval (wave3d_eval.cpp:299-415) — every call is a
+   fresh evaluation (the reference's eval is single-shot). */
+int ddfmm_eval(ddfmm_ctx* ctx, const double* den, double* val, int nrhs);
+
+/* Same with device-resident buffers on `stream` (0 = the context's stream).
+   Asynchronous with respect to the host. */
+int ddfmm_eval_device(ddfmm_ctx* ctx, const void* den_dev, void* val_dev, int nrhs, void* stream);
+
+/* Direct FP64 sum at targets (Wave3d::check's oracle sum, wave3d_check.cpp:66-76):
+   out[r][i] = sum_j G(trg_i, src_j) den[r][j]; host buffers; points as rows. */
+int ddfmm_direct_sum(int kernel, const double* trg, int m, const double* src, int n,
+                     const double* den, int nrhs, double* out);
+
+/* Dense kernel block (Kernel3d::kernel, kernel3d.cpp:26-107): out(i,j) = G(trg_i, src_j),
+   m x n column-major complex; points as rows. */
+int ddfmm_kernel_matrix(int kernel, const double* trg, int m, const double* src, int n, double* out);
+
+int ddfmm_get_stats(ddfmm_ctx* ctx, ddfmm_stats* st);
+/* Canonical text of every box's lists and direction sets (integer parity checks). */
+int ddfmm_dump_lists(ddfmm_ctx* ctx, char* buf, int64_t cap, int64_t* len);
+/* Per-stage timing: enable, then read names and milliseconds of the last eval. */
+int ddfmm_set_timing(ddfmm_ctx* ctx, int on);
+int ddfmm_stage_times(ddfmm_ctx* ctx, int cap, char* names /* cap x 24 */, double* ms, int* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -566,3 +566,29 @@ class Wave3dOracle:
                     dnchk[ck] = dnchk.get(ck, 0) + de2dc[ind] @ dneq
                 else:
                     dirdn[(ck, pd)] = dirdn.get((ck, pd), 0) + de2dc[ind] @ dneq
+
+
+def dump_lists(w: "Wave3dOracle") -> str:
+    """Canonical text of every non-empty box: tag, point count, fftnum, U/V/W/X/E lists,
+    far lists by direction and the in/out direction sets (same layout as the C ABI's
+    ddfmm_dump_lists)."""
+    out = []
+
+    def key(k):
+        return f"{k[0]}:{k[1]},{k[2]},{k[3]}"
+
+    def dr(d):
+        return f"{d[0]},{d[1]},{d[2]}"
+
+    for k, b in w.boxes.items():
+        if not (b.tag & PTS):
+            continue
+        s = f"{key(k)} t{b.tag} n{len(b.ptidx)} f{b.fftnum}"
+        for nm, lst in zip("UVWXE", (b.U, b.V, b.W, b.X, b.E)):
+            s += f" {nm}[" + "".join(key(x) + " " for x in lst) + "]"
+        for d in sorted(b.fnde):
+            s += f" F({dr(d)})[" + "".join(key(x) + " " for x in b.fnde[d]) + "]"
+        s += " I[" + "".join(dr(d) + " " for d in sorted(b.incdir)) + "]"
+        s += " O[" + "".join(dr(d) + " " for d in sorted(b.outdir)) + "]"
+        out.append(s + "\n")
+    return "".join(out)

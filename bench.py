@@ -1,0 +1,209 @@
+#!/usr/bin/env python3
+"""Throughput of the directional-FMM evaluation pass on one MI355X.
+
+One step = one Wave3d::eval over the workload (densities and potentials resident in
+HBM, eval_device).  Default workload: sphere of diameter 32 wavelengths, N = 1,000,000
+area-uniform points (seed 1), one complex-normal density vector, kernel exp(ikr)/r,
+ACCU=2 (P=6), NPQ=4, ptsmax=100.  Metric: M point-evaluations/s per RHS, reported
+with the relative L2 error at 1000 random targets against an FP64 direct sum.
+
+Prints one JSON line.  Also reports the roofline of the dominant kernel (HIP events on
+the evaluation stream) and a CPU baseline (the oracle's numpy FMM on a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "hbm": 8.0e12}
+KERNEL_OF_STAGE = {"lf_s2m": "kapply", "lf_x2chk": "kapply", "leaf_near_l2t": "kapply",
+                   "lf_fft": "fft_fwd", "lf_vlist": "vlist", "gather": "gather", "scatter": "scatter"}
+
+
+def kernel_of(stage: str) -> str:
+    if stage in KERNEL_OF_STAGE:
+        return KERNEL_OF_STAGE[stage]
+    if stage.startswith("zero"):
+        return "zero"
+    return "gemm"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(ops_paths, kernel):
+    """The oracle (numpy FP64 FMM) on the 60k-point sphere of diameter 8 (K = 8)."""
+    from threadpoolctl import threadpool_limits
+
+    from ddfmm_amd import geometry, serialize
+    from oracle import wave3d_ref
+
+    cores = min(16, len(os.sched_getaffinity(0)))
+    N, K = 60_000, 8.0
+    pos = geometry.sphere(N, K, 0)
+    den = geometry.densities(N, 0)
+    ld, hd = serialize.read_ld(ops_paths[0]), serialize.read_hd(ops_paths[1])
+    with threadpool_limits(limits=cores):
+        w = wave3d_ref.Wave3dOracle(pos, K, ld, hd, ktype=kernel).setup()
+        t0 = time.perf_counter()
+        w.eval(den)
+        dt = time.perf_counter() - t0
+    return {"value": N / dt / 1e6, "unit": "M point-evals/s·RHS", "cores": cores, "kind": "port",
+            "sample": f"oracle numpy FMM eval (FP64) on the sphere d=8 N=60000 K=8 workload, "
+                      f"one RHS, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="sphere32_1m")
+    ap.add_argument("--kernel", type=int, default=0)
+    ap.add_argument("--nrhs", type=int, default=None)
+    ap.add_argument("--precision", default="fp64", choices=["fp64"])
+    ap.add_argument("--chk", type=int, default=1000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(one device share per process, no collectives).", file=sys.stderr)
+        sys.exit(2)
+
+    from ddfmm_amd import geometry, mlib_gen
+    from ddfmm_amd._lib import Hip
+    from ddfmm_amd.wave3d import FP64, Kernel3d, Mlib3d, Wave3d, direct_sum
+
+    wl = dict(geometry.WORKLOADS[args.workload])
+    nrhs = args.nrhs or wl["nrhs"]
+    ops = mlib_gen.ensure_library(os.path.join(ROOT, "ddfmm_amd", "_ops"), ktype=args.kernel, accu=2, NPQ=4,
+                                  hf_max_W=4, lf_min_W=2.0 ** -5)
+    t0 = time.perf_counter()
+    pos = geometry.workload_points(args.workload)
+    den = geometry.densities(len(pos), wl["seed"], nrhs=nrhs)
+    N = len(pos)
+    log(f"workload {args.workload}: N={N} nrhs={nrhs} kernel={args.kernel} ({time.perf_counter() - t0:.1f}s)")
+
+    ml = Mlib3d()
+    ml.setup({"-mlib3d_ldname": ops[0], "-mlib3d_hdname": ops[1]})
+    w = Wave3d()
+    w.posptr, w.mlibptr, w.kernel = pos, ml, Kernel3d(args.kernel)
+    w.geomprtn = np.zeros((wl.get("numC", 1),) * 3, dtype=np.int32)
+    t0 = time.perf_counter()
+    w.setup({"-wave3d_ACCU": 2, "-wave3d_NPQ": 4, "-wave3d_K": wl["K"], "-wave3d_ptsmax": wl["ptsmax"],
+             "-wave3d_maxlevel": 14}, nrhs_max=nrhs, precision=FP64)
+    st = w.stats()
+    log(f"setup {time.perf_counter() - t0:.1f}s: leaves={st['nleaf']} m2l_pairs={st['m2l_pairs']} "
+        f"v_pairs={st['v_pairs']} device={st['device_bytes'] / 2**30:.2f} GiB")
+
+    hip = Hip()
+    den2 = np.ascontiguousarray(np.atleast_2d(den))
+    dd = hip.to_device(den2)
+    vd = hip.malloc(den2.nbytes)
+    for _ in range(args.warmup):
+        w.eval_device(dd, vd, nrhs)
+    hip.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        w.eval_device(dd, vd, nrhs)
+    hip.sync()
+    dt = (time.perf_counter() - t0) / args.steps
+    log(f"{args.steps} steps: {dt * 1e3:.3f} ms/step")
+
+    # accuracy at random targets vs the FP64 direct sum
+    val = hip.to_host(vd, np.empty_like(den2))
+    chk = geometry.check_targets(N, args.chk, wl["seed"])
+    exact = direct_sum(pos[chk], pos, den2, args.kernel)
+    err = float(np.linalg.norm(val[:, chk] - exact) / np.linalg.norm(exact))
+    log(f"rel.err vs FP64 direct at {len(chk)} targets: {err:.3e}")
+
+    # per-kernel time from HIP events on the evaluation stream
+    w.set_timing(True)
+    per = {}
+    reps = 3
+    for _ in range(reps):
+        w.eval_device(dd, vd, nrhs)
+        for name, ms in w.stage_times():
+            k = kernel_of(name)
+            per.setdefault(k, [0.0, 0])
+            per[k][0] += ms / reps
+            per[k][1] += 1
+    for k in per:
+        per[k][1] //= reps
+    w.set_timing(False)
+    dom = max(per, key=lambda k: per[k][0])
+    t_dom = per[dom][0] * 1e-3
+    launches = per[dom][1]
+    log("per kernel (ms/eval, launches): " + ", ".join(f"{k}={v[0]:.3f}/{v[1]}" for k, v in
+                                                        sorted(per.items(), key=lambda kv: -kv[1][0])))
+    if dom == "gemm":
+        work = st["gemm_flops"] * nrhs
+        roof = {"bound": "mfma", "achieved": work / t_dom / 1e12, "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s"}
+        work_desc = "useful complex-GEMM flops 8*M*K*N summed over all groups"
+    elif dom == "kapply":
+        inter = st["near_interactions"] + st["leaf_op_interactions"] + st["s2m_interactions"] + st["x2chk_interactions"]
+        work = inter * (20 + 8 * nrhs)
+        roof = {"bound": "mfma", "achieved": work / t_dom / 1e12, "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s"}
+        work_desc = "kernel interactions x (20 + 8*nrhs) flops"
+    elif dom == "vlist":
+        work = (st["v_sources"] + st["v_targets"]) * 1728 * 16 * nrhs
+        roof = {"bound": "hbm", "achieved": work / t_dom / 1e9, "peak": PEAK["hbm"] / 1e9, "unit": "GB/s"}
+        work_desc = "compulsory spectrum bytes (sources + targets) x 1728 x 16 B"
+    else:
+        work = 0
+        roof = {"bound": "hbm", "achieved": 0.0, "peak": PEAK["hbm"] / 1e9, "unit": "GB/s"}
+        work_desc = "n/a"
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = None
+    roof["kernel"] = {"gemm": "gemm_kernel", "kapply": "kapply_kernel", "vlist": "vlist_kernel",
+                      "fft_fwd": "fft_fwd_kernel"}.get(dom, dom)
+    roof["launches_per_eval"] = launches
+    roof["avg_launch_us"] = t_dom / max(launches, 1) * 1e6
+    roof["work"] = work_desc
+    hip.free(dd)
+    hip.free(vd)
+    w.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle) ...")
+        cpu = cpu_baseline(mlib_gen.library_paths(os.path.join(ROOT, "ddfmm_amd", "_ops"), args.kernel, 2, 4),
+                           args.kernel)
+        log(f"cpu baseline {cpu['value']:.3f} M/s on {cpu['cores']} cores")
+
+    value = N * nrhs / dt / 1e6
+    out = {
+        "metric": "M point-evals/s·RHS at rel.err<=1e-5 vs FP64 direct",
+        "value": value,
+        "unit": "M point-evals/s·RHS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.precision,
+        "data": "synthetic",
+        "config": {"workload": args.workload, "n_points": N, "nrhs": nrhs, "kernel": "exp(ikr)/r" if args.kernel == 0
+                   else "exp(ikr)", "K": wl["K"], "ACCU": 2, "NPQ": 4, "ptsmax": wl["ptsmax"],
+                   "precision": args.precision, "rel_err_vs_direct": err, "n_check": int(len(chk)),
+                   "meets_tolerance": bool(err <= 1e-5)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "per_kernel_ms": {k: round(v[0], 4) for k, v in per.items()},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

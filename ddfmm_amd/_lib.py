@@ -11,7 +11,7 @@ LIB_PATH = os.path.join(HERE, "libddfmm.so")
 
 EXPORTS = ["ddfmm_last_error", "ddfmm_create", "ddfmm_destroy", "ddfmm_setup", "ddfmm_eval",
            "ddfmm_eval_device", "ddfmm_direct_sum", "ddfmm_kernel_matrix", "ddfmm_get_stats",
-           "ddfmm_dump_lists", "ddfmm_set_timing", "ddfmm_stage_times"]
+           "ddfmm_dump_lists", "ddfmm_read_buffer", "ddfmm_set_timing", "ddfmm_stage_times"]
 
 
 class Params(ctypes.Structure):
@@ -60,6 +60,7 @@ def lib():
     L.ddfmm_kernel_matrix.argtypes = [ci, vp, ci, vp, ci, vp]
     L.ddfmm_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
     L.ddfmm_dump_lists.argtypes = [vp, ctypes.c_char_p, i64, ctypes.POINTER(i64)]
+    L.ddfmm_read_buffer.argtypes = [vp, ci, ci, i64, i64, vp]
     L.ddfmm_set_timing.argtypes = [vp, ci]
     L.ddfmm_stage_times.argtypes = [vp, ci, ctypes.c_char_p, ctypes.POINTER(cd), ctypes.POINTER(ci)]
     for name in EXPORTS:
@@ -76,3 +77,46 @@ def check(rc: int):
 
 def ptr(a: np.ndarray) -> int:
     return a.ctypes.data
+
+
+class Hip:
+    """Minimal HIP runtime calls (the same libamdhip64 that libddfmm.so links) for
+    device-resident buffers and device-wide synchronisation in tests and benchmarks.
+    torch is not used in processes that load libddfmm.so: its wheel bundles a different
+    HIP runtime."""
+
+    def __init__(self):
+        lib()
+        self.rt = ctypes.CDLL("libamdhip64.so.7")
+        vp = ctypes.c_void_p
+        self.rt.hipMalloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t]
+        self.rt.hipFree.argtypes = [vp]
+        self.rt.hipMemcpy.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int]
+        self.rt.hipDeviceSynchronize.argtypes = []
+        for f in ("hipMalloc", "hipFree", "hipMemcpy", "hipDeviceSynchronize"):
+            getattr(self.rt, f).restype = ctypes.c_int
+
+    def _ok(self, rc, what):
+        if rc != 0:
+            raise DdfmmError(f"{what} failed with HIP error {rc}")
+
+    def malloc(self, nbytes: int) -> int:
+        p = ctypes.c_void_p()
+        self._ok(self.rt.hipMalloc(ctypes.byref(p), nbytes), "hipMalloc")
+        return p.value
+
+    def free(self, p: int):
+        self._ok(self.rt.hipFree(p), "hipFree")
+
+    def to_device(self, a: np.ndarray) -> int:
+        a = np.ascontiguousarray(a)
+        p = self.malloc(a.nbytes)
+        self._ok(self.rt.hipMemcpy(p, a.ctypes.data, a.nbytes, 1), "hipMemcpy H2D")
+        return p
+
+    def to_host(self, p: int, out: np.ndarray) -> np.ndarray:
+        self._ok(self.rt.hipMemcpy(out.ctypes.data, p, out.nbytes, 2), "hipMemcpy D2H")
+        return out
+
+    def sync(self):
+        self._ok(self.rt.hipDeviceSynchronize(), "hipDeviceSynchronize")

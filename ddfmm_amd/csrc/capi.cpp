@@ -134,7 +134,7 @@ int ddfmm_setup(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_params* 
         c->precision = p->precision;
         Mlib mlib;
         mlib.load(ldfile, hdfile);
-        build_plan(q, pos, n, mlib, c->hp, n <= 2000000);
+        build_plan(q, pos, n, mlib, c->hp, c->device < 0);
     } catch (const HipFail& e) {
         return fail(DDFMM_EDEVICE, e.what());
     } catch (const std::exception& e) {
@@ -330,6 +330,24 @@ int ddfmm_dump_lists(ddfmm_ctx* c, char* buf, int64_t cap, int64_t* len) {
     if (!c->ready) return fail(DDFMM_ESTATE, "context not set up");
     *len = (int64_t)c->hp.dump.size();
     if (buf && cap > 0) std::memcpy(buf, c->hp.dump.data(), (size_t)std::min<int64_t>(cap, *len));
+    return DDFMM_OK;
+}
+
+int ddfmm_read_buffer(ddfmm_ctx* c, int buf, int rhs, int64_t off, int64_t count, double* out) {
+    if (!c || !out) return fail(DDFMM_EINVAL, "null argument");
+    if (!c->ready || c->device < 0) return fail(DDFMM_ESTATE, "context not set up on a device");
+    if (buf < 0 || buf >= BUF_COUNT || rhs < 0 || rhs >= c->nrhs_max || off < 0 || count < 0 ||
+        off + count > c->hp.bufsize[buf])
+        return fail(DDFMM_EINVAL, "buffer range out of bounds");
+    if (count == 0) return DDFMM_OK;
+    try {
+        HIPCHK(hipSetDevice(c->device));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpy(out, c->dp.bt.p[buf] + (int64_t)rhs * c->hp.bufsize[buf] + off,
+                         (size_t)count * sizeof(double2), hipMemcpyDeviceToHost));
+    } catch (const std::exception& e) {
+        return fail(DDFMM_EDEVICE, e.what());
+    }
     return DDFMM_OK;
 }
 

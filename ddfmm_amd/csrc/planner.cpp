@@ -113,6 +113,7 @@ struct HfDir {          // one (width, direction) group
     int r1, r2, r1c;    // equivalent / check / child-equivalent sizes
     int64_t uep_g, ucp_g, uepc_g;   // generator point offsets (shuffled)
     int64_t ue2uc[8];   // generated matrices (relative offsets)
+    int64_t de2dc[8];   // W = 1 only: G(uep_LF(W/2) + s_c, -ucp) (not a transpose of ue2uc)
     int64_t e1, e2, e3;
     std::vector<BoxKey> out, in;
 };
@@ -294,6 +295,14 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
                 double s[3];
                 shift_of(c, W, s);
                 h.ue2uc[c] = gen(h.ucp_g, h.r2, zero, 0, h.uepc_g, h.r1c, s);
+                h.de2dc[c] = -1;
+                if (W == 1.0) {
+                    // DownwardHighFetch at W = 1 (mlib3d.cpp:262-265): child check points are
+                    // the LF equivalent surface, not negated; de2dc(c) = G(uepc + s_c, -ucp),
+                    // evaluated as G((-uepc - s_c) - ucp).
+                    double ms[3] = {-s[0], -s[1], -s[2]};
+                    h.de2dc[c] = gen(h.uepc_g, h.r1c, ms, 1, h.ucp_g, h.r2, zero);
+                }
             }
         }
         scr_size = std::max(scr_size, std::max(lvl_o, lvl_i));
@@ -512,9 +521,15 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
                         cc.push_back(it->second);
                     }
                 }
-                // de2dc(c) = G(-uepc + s_c, -ucp) = ue2uc(7 - c)^T  (mlib3d.cpp:227-280)
+                // W > 1: de2dc(c) = G(-uepc + s_c, -ucp) = ue2uc(7 - c)^T (mlib3d.cpp:227-280);
+                // W = 1: explicit matrix (r1c x r2)
                 GGroup g{};
-                g.aoff = G_(h.ue2uc[7 - c]); g.soff = -1; g.M = h.r1c; g.K = h.r2; g.lda = h.r2; g.transA = 1;
+                if (W == 1.0) {
+                    g.aoff = G_(h.de2dc[c]); g.M = h.r1c; g.K = h.r2; g.lda = h.r1c; g.transA = 0;
+                } else {
+                    g.aoff = G_(h.ue2uc[7 - c]); g.M = h.r1c; g.K = h.r2; g.lda = h.r2; g.transA = 1;
+                }
+                g.soff = -1;
                 g.bbuf = BUF_HF_A; g.cbuf = W == 1.0 ? BUF_LF_CHK : BUF_HF_DN; g.mode = 2;
                 add_group(g, bc, cc, nullptr);
             }

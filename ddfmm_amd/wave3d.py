@@ -154,6 +154,11 @@ class Wave3d:
         check(lib().ddfmm_dump_lists(self._ctx, buf, n.value, ctypes.byref(n)))
         return buf.raw[:n.value].decode()
 
+    def read_buffer(self, buf: int, off: int, count: int, rhs: int = 0) -> np.ndarray:
+        out = np.empty(count, dtype=np.complex128)
+        check(lib().ddfmm_read_buffer(self._ctx, buf, rhs, off, count, ptr(out)))
+        return out
+
     def set_timing(self, on: bool = True):
         check(lib().ddfmm_set_timing(self._ctx, int(on)))
 

@@ -80,6 +80,10 @@ int ddfmm_kernel_matrix(int kernel, const double* trg, int m, const double* src,
 int ddfmm_get_stats(ddfmm_ctx* ctx, ddfmm_stats* st);
 /* Canonical text of every box's lists and direction sets (integer parity checks). */
 int ddfmm_dump_lists(ddfmm_ctx* ctx, char* buf, int64_t cap, int64_t* len);
+/* Inspection: copy `count` complex entries of internal vector buffer `buf` (leaf-sorted
+   densities/potentials, LF/HF equivalent densities and check values; numbering of
+   ddfmm_amd/csrc/plan.hpp) for right-hand side `rhs`, starting at `off`. */
+int ddfmm_read_buffer(ddfmm_ctx* ctx, int buf, int rhs, int64_t off, int64_t count, double* out);
 /* Per-stage timing: enable, then read names and milliseconds of the last eval. */
 int ddfmm_set_timing(ddfmm_ctx* ctx, int on);
 int ddfmm_stage_times(ddfmm_ctx* ctx, int cap, char* names /* cap x 24 */, double* ms, int* count);

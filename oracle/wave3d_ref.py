@@ -466,6 +466,7 @@ class Wave3dOracle:
             dn_rec(1, d)
         # LF downward, coarsest first (wave3d_eval.cpp:92-104, 493-583)
         fftcache = {}
+        dneq_all = {}
         for W in sorted(ldmap, reverse=True):
             uep, ucp, (V, invs, UH), ue2dc = self._lf(W)
             dep, dcp = ucp, uep
@@ -508,6 +509,7 @@ class Wave3dOracle:
                     else:
                         chk = chk + kernel(dcp + c, extpos[nk], kt) @ extden[nk]
                 dneq = UH.T @ ((V.T @ chk) * invs[:, 0])
+                dneq_all[k] = dneq
                 if term:
                     ev = ev + kernel(tp, dep + c, kt) @ dneq
                     extval[k] = ev
@@ -521,6 +523,8 @@ class Wave3dOracle:
         val = np.zeros(len(self.pos), dtype=np.complex128)
         for k, ev in extval.items():
             val[boxes[k].ptidx] = ev
+        # intermediate quantities, kept for stage-by-stage comparisons
+        self.upeq, self.dirup, self.dirdn, self.dneq, self.hdmap = upeq, dirup, dirdn, dneq_all, hdmap
         return val
 
     def _up_high(self, W, d, lists, boxes, upeq, dirup):
@@ -557,6 +561,7 @@ class Wave3dOracle:
                 if off not in mcache:
                     mcache[off] = kernel(dcp + np.asarray(off, dtype=np.float64) * W, uep, kt)
                 dcv = dcv + mcache[off] @ dirup.get((sk, d), np.zeros(len(uep), dtype=np.complex128))
+            dirdn[(k, d)] = dcv
             dneq = E3.T @ (E2.T @ (E1.T @ dcv))
             for ind in range(8):
                 ck = child_key(k, ind)

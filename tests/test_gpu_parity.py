@@ -132,17 +132,19 @@ def test_expr_kernel_vs_oracle(oplib_expr, ops_expr):
 
 
 def test_eval_device_buffers(oplib):
-    torch = pytest.importorskip("torch")
+    from ddfmm_amd._lib import Hip
     K, N = 4.0, 4000
     pos = geometry.sphere(N, K, 5)
     den = geometry.densities(N, 5, nrhs=2)
     w = _wave(pos, K, oplib, nrhs_max=2)
     host = w.eval(den)
-    d = torch.from_numpy(den).to("cuda:0")
-    v = torch.empty_like(d)
-    torch.cuda.synchronize()
-    w.eval_device(d.data_ptr(), v.data_ptr(), 2, 0)
-    torch.cuda.synchronize()       # device-wide: covers the context's own stream
-    got = v.cpu().numpy()
+    hip = Hip()
+    d = hip.to_device(den)
+    v = hip.malloc(den.nbytes)
+    w.eval_device(d, v, 2, 0)
+    hip.sync()
+    got = hip.to_host(v, np.empty_like(den))
+    hip.free(d)
+    hip.free(v)
     assert rel(got, host) < 1e-12
     w.close()

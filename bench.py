@@ -24,16 +24,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "hbm": 8.0e12}
-KERNEL_OF_STAGE = {"lf_s2m": "kapply", "lf_x2chk": "kapply", "leaf_near_l2t": "kapply",
-                   "lf_fft": "fft_fwd", "lf_vlist": "vlist", "gather": "gather", "scatter": "scatter"}
+ST_ZERO, ST_GATHER, ST_SCATTER, ST_KAPPLY, ST_GEMM, ST_FFTFWD, ST_VLIST = range(7)
 
 
-def kernel_of(stage: str) -> str:
-    if stage in KERNEL_OF_STAGE:
-        return KERNEL_OF_STAGE[stage]
-    if stage.startswith("zero"):
-        return "zero"
-    return "gemm"
+def kernel_class(kind: int, prec: int) -> str:
+    if kind == ST_GEMM:
+        return "gemm_solve" if prec == 1 else "gemm_translation"
+    return {ST_KAPPLY: "kapply", ST_VLIST: "vlist", ST_FFTFWD: "fft_fwd", ST_ZERO: "zero",
+            ST_GATHER: "gather", ST_SCATTER: "scatter"}[kind]
 
 
 def log(msg):
@@ -70,7 +68,9 @@ def main():
     ap.add_argument("--workload", default="sphere32_1m")
     ap.add_argument("--kernel", type=int, default=0)
     ap.add_argument("--nrhs", type=int, default=None)
-    ap.add_argument("--precision", default="fp64", choices=["fp64"])
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="one 256 MiB device-to-device copy (blit kernel) to calibrate PMC byte counters")
     ap.add_argument("--chk", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -81,12 +81,12 @@ def main():
 
     from ddfmm_amd import geometry, mlib_gen
     from ddfmm_amd._lib import Hip
-    from ddfmm_amd.wave3d import FP64, Kernel3d, Mlib3d, Wave3d, direct_sum
+    from ddfmm_amd.wave3d import FP64, MIXED, Kernel3d, Mlib3d, Wave3d, direct_sum
 
     wl = dict(geometry.WORKLOADS[args.workload])
     nrhs = args.nrhs or wl["nrhs"]
     ops = mlib_gen.ensure_library(os.path.join(ROOT, "ddfmm_amd", "_ops"), ktype=args.kernel, accu=2, NPQ=4,
-                                  hf_max_W=4, lf_min_W=2.0 ** -5)
+                                  hf_max_W=4, lf_min_W=2.0 ** -3)
     t0 = time.perf_counter()
     pos = geometry.workload_points(args.workload)
     den = geometry.densities(len(pos), wl["seed"], nrhs=nrhs)
@@ -100,12 +100,19 @@ def main():
     w.geomprtn = np.zeros((wl.get("numC", 1),) * 3, dtype=np.int32)
     t0 = time.perf_counter()
     w.setup({"-wave3d_ACCU": 2, "-wave3d_NPQ": 4, "-wave3d_K": wl["K"], "-wave3d_ptsmax": wl["ptsmax"],
-             "-wave3d_maxlevel": 14}, nrhs_max=nrhs, precision=FP64)
+             "-wave3d_maxlevel": 14}, nrhs_max=nrhs, precision=MIXED if args.precision == "mixed" else FP64)
     st = w.stats()
     log(f"setup {time.perf_counter() - t0:.1f}s: leaves={st['nleaf']} m2l_pairs={st['m2l_pairs']} "
         f"v_pairs={st['v_pairs']} device={st['device_bytes'] / 2**30:.2f} GiB")
 
     hip = Hip()
+    if args.calib_copy:
+        nb = 256 << 20
+        a, b = hip.malloc(nb), hip.malloc(nb)
+        hip.copy_d2d(b, a, nb)
+        hip.sync()
+        hip.free(a)
+        hip.free(b)
     den2 = np.ascontiguousarray(np.atleast_2d(den))
     dd = hip.to_device(den2)
     vd = hip.malloc(den2.nbytes)
@@ -132,40 +139,49 @@ def main():
     reps = 3
     for _ in range(reps):
         w.eval_device(dd, vd, nrhs)
-        for name, ms in w.stage_times():
-            k = kernel_of(name)
-            per.setdefault(k, [0.0, 0])
-            per[k][0] += ms / reps
-            per[k][1] += 1
+        for name, ms, work, kind, prec in w.stage_times():
+            k = kernel_class(kind, prec)
+            e = per.setdefault(k, {"ms": 0.0, "launches": 0, "work": 0.0})
+            e["ms"] += ms / reps
+            e["launches"] += 1
+            e["work"] += work * nrhs / reps
     for k in per:
-        per[k][1] //= reps
+        per[k]["launches"] //= reps
     w.set_timing(False)
-    dom = max(per, key=lambda k: per[k][0])
-    t_dom = per[dom][0] * 1e-3
-    launches = per[dom][1]
-    log("per kernel (ms/eval, launches): " + ", ".join(f"{k}={v[0]:.3f}/{v[1]}" for k, v in
-                                                        sorted(per.items(), key=lambda kv: -kv[1][0])))
-    if dom == "gemm":
-        work = st["gemm_flops"] * nrhs
-        roof = {"bound": "mfma", "achieved": work / t_dom / 1e12, "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s"}
-        work_desc = "useful complex-GEMM flops 8*M*K*N summed over all groups"
+    dom = max(per, key=lambda k: per[k]["ms"])
+    t_dom = per[dom]["ms"] * 1e-3
+    launches = per[dom]["launches"]
+    log("per kernel class (ms/eval, launches): " + ", ".join(
+        f"{k}={v['ms']:.3f}/{v['launches']}" for k, v in sorted(per.items(), key=lambda kv: -kv[1]["ms"])))
+    mixed = args.precision == "mixed"
+    esz = 8 if mixed else 16
+    kname = {"gemm_translation": "gemm_kernel<float2,float2,float>" if mixed else "gemm_kernel<double2,double2,double>",
+             "gemm_solve": "gemm_kernel<double2,float2,double>" if mixed else "gemm_kernel<double2,double2,double>",
+             "kapply": "kapply_kernel", "vlist": "vlist_kernel<12>", "fft_fwd": "fft_fwd_kernel<12>"}.get(dom, dom)
+    if dom in ("gemm_translation", "gemm_solve"):
+        peak = PEAK["fp32"] if (mixed and dom == "gemm_translation") else PEAK["fp64"]
+        roof = {"bound": "mfma", "achieved": per[dom]["work"] / t_dom / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s"}
+        work_desc = "useful complex-GEMM flops 8*M*K*N over the class's groups"
     elif dom == "kapply":
-        inter = st["near_interactions"] + st["leaf_op_interactions"] + st["s2m_interactions"] + st["x2chk_interactions"]
-        work = inter * (20 + 8 * nrhs)
-        roof = {"bound": "mfma", "achieved": work / t_dom / 1e12, "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s"}
-        work_desc = "kernel interactions x (20 + 8*nrhs) flops"
+        work = per[dom]["work"] / nrhs * (20 + 8 * nrhs)
+        peak = PEAK["fp32"] if mixed else PEAK["fp64"]
+        roof = {"bound": "mfma", "achieved": work / t_dom / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s"}
+        work_desc = "target-source interactions x (20 + 8*nrhs) flops (VALU)"
     elif dom == "vlist":
-        work = (st["v_sources"] + st["v_targets"]) * 1728 * 16 * nrhs
+        work = ((st["v_sources"] + st["v_targets"]) * 1728 * esz * nrhs + st["nlevels_lf"] * 316 * 1728 * esz)
         roof = {"bound": "hbm", "achieved": work / t_dom / 1e9, "peak": PEAK["hbm"] / 1e9, "unit": "GB/s"}
-        work_desc = "compulsory spectrum bytes (sources + targets) x 1728 x 16 B"
+        work_desc = "compulsory bytes: (V sources + V targets) x 1728 spectra entries + translation tables"
     else:
-        work = 0
         roof = {"bound": "hbm", "achieved": 0.0, "peak": PEAK["hbm"] / 1e9, "unit": "GB/s"}
         work_desc = "n/a"
     roof["frac"] = roof["achieved"] / roof["peak"]
     roof["traffic"] = None
-    roof["kernel"] = {"gemm": "gemm_kernel", "kapply": "kapply_kernel", "vlist": "vlist_kernel",
-                      "fft_fwd": "fft_fwd_kernel"}.get(dom, dom)
+    tfile = os.path.join(ROOT, "profiles", f"{args.workload}_{args.precision}_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            roof["traffic"] = json.load(f).get(kname)
+    roof["kernel"] = kname
+    roof["class"] = dom
     roof["launches_per_eval"] = launches
     roof["avg_launch_us"] = t_dom / max(launches, 1) * 1e6
     roof["work"] = work_desc
@@ -192,7 +208,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": args.precision,
+        "dtype": "fp32+fp64 (FP32 translations/near field, FP64 solves)" if args.precision == "mixed" else "fp64",
         "data": "synthetic",
         "config": {"workload": args.workload, "n_points": N, "nrhs": nrhs, "kernel": "exp(ikr)/r" if args.kernel == 0
                    else "exp(ikr)", "K": wl["K"], "ACCU": 2, "NPQ": 4, "ptsmax": wl["ptsmax"],
@@ -200,7 +216,7 @@ def main():
                    "meets_tolerance": bool(err <= 1e-5)},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "per_kernel_ms": {k: round(v[0], 4) for k, v in per.items()},
+        "per_kernel_ms": {k: round(v["ms"], 4) for k, v in per.items()},
     }
     print(json.dumps(out), flush=True)
 

@@ -62,7 +62,8 @@ def lib():
     L.ddfmm_dump_lists.argtypes = [vp, ctypes.c_char_p, i64, ctypes.POINTER(i64)]
     L.ddfmm_read_buffer.argtypes = [vp, ci, ci, i64, i64, vp]
     L.ddfmm_set_timing.argtypes = [vp, ci]
-    L.ddfmm_stage_times.argtypes = [vp, ci, ctypes.c_char_p, ctypes.POINTER(cd), ctypes.POINTER(ci)]
+    L.ddfmm_stage_times.argtypes = [vp, ci, ctypes.c_char_p, ctypes.POINTER(cd), ctypes.POINTER(cd),
+                                     ctypes.POINTER(ci), ctypes.POINTER(ci)]
     for name in EXPORTS:
         if name != "ddfmm_last_error":
             getattr(L, name).restype = ci
@@ -117,6 +118,9 @@ class Hip:
     def to_host(self, p: int, out: np.ndarray) -> np.ndarray:
         self._ok(self.rt.hipMemcpy(out.ctypes.data, p, out.nbytes, 2), "hipMemcpy D2H")
         return out
+
+    def copy_d2d(self, dst: int, src: int, nbytes: int):
+        self._ok(self.rt.hipMemcpy(dst, src, nbytes, 3), "hipMemcpy D2D")
 
     def sync(self):
         self._ok(self.rt.hipDeviceSynchronize(), "hipDeviceSynchronize")

@@ -115,7 +115,8 @@ int ddfmm_setup(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_params* 
     if (p->ACCU < 1 || p->ACCU > 3) return fail(DDFMM_EINVAL, "ACCU must be 1, 2 or 3");
     if (p->ACCU != 2) return fail(DDFMM_EINVAL, "this build evaluates ACCU=2 (P=6) only");
     if (p->nrhs_max < 1) return fail(DDFMM_EINVAL, "nrhs_max must be >= 1");
-    if (p->precision != DDFMM_FP64) return fail(DDFMM_EINVAL, "unsupported precision mode");
+    if (p->precision != DDFMM_FP64 && p->precision != DDFMM_MIXED)
+        return fail(DDFMM_EINVAL, "unsupported precision mode");
     double lk = std::log2(p->K);
     if (!(p->K >= 1) || std::abs(lk - std::round(lk)) > 1e-12) return fail(DDFMM_EINVAL, "K must be a power of two >= 1");
     try {
@@ -149,6 +150,8 @@ int ddfmm_setup(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_params* 
         DevPlan& d = c->dp;
         d = DevPlan();
         d.N = h.N; d.P = h.P; d.NL = h.NL; d.ktype = c->prm.ktype;
+        d.mixed = c->precision == DDFMM_MIXED ? 1 : 0;
+        const int64_t esz = d.mixed ? (int64_t)sizeof(float2) : (int64_t)sizeof(double2);
         d.px = c->upload(h.px); d.py = c->upload(h.py); d.pz = c->upload(h.pz);
         d.perm = c->upload(h.perm);
         d.grid = c->upload(h.grid);
@@ -161,7 +164,8 @@ int ddfmm_setup(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_params* 
         d.fsrc = c->upload(h.fsrc); d.vtrg = c->upload(h.vtrg); d.vpair = c->upload(h.vpair);
         for (int b = 0; b < BUF_COUNT; ++b) {
             d.bt.stride[b] = h.bufsize[b];
-            d.bt.p[b] = c->alloc<double2>(h.bufsize[b] * c->nrhs_max);
+            d.bt.p[b] = esz == 8 ? (void*)c->alloc<float2>(h.bufsize[b] * c->nrhs_max)
+                                 : (void*)c->alloc<double2>(h.bufsize[b] * c->nrhs_max);
         }
         d.den_io = c->alloc<double2>(h.N * c->nrhs_max);
         d.val_io = c->alloc<double2>(h.N * c->nrhs_max);
@@ -177,6 +181,11 @@ int ddfmm_setup(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_params* 
             HIPCHK(hipStreamSynchronize(c->stream));
             HIPCHK(hipFree(gp));
             HIPCHK(hipFree(jobs));
+        }
+        if (d.mixed) {
+            d.opmat32 = c->alloc<float2>(h.opmat_total);
+            HIPCHK(launch_to_f32(d.opmat, d.opmat32, h.opmat_total, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
         }
         // host copies no longer needed on the device path
         std::vector<cpx>().swap(h.opmat);
@@ -299,7 +308,7 @@ int ddfmm_kernel_matrix(int kernel, const double* trg, int m, const double* src,
         HIPCHK(hipMalloc(&dout, (size_t)m * n * sizeof(double2)));
         HIPCHK(hipMemcpy(dt, trg, (size_t)m * 3 * sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(ds, src, (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice));
-        HIPCHK(launch_kmatrix(dt, m, ds, n, dout, kernel, nullptr));
+        HIPCHK(launch_kmatrix(dt, m, ds, n, dout, kernel, 0, nullptr));
         HIPCHK(hipMemcpy(out, dout, (size_t)m * n * sizeof(double2), hipMemcpyDeviceToHost));
     } catch (const std::exception& e) {
         rc = fail(DDFMM_EDEVICE, e.what());
@@ -343,8 +352,15 @@ int ddfmm_read_buffer(ddfmm_ctx* c, int buf, int rhs, int64_t off, int64_t count
     try {
         HIPCHK(hipSetDevice(c->device));
         HIPCHK(hipStreamSynchronize(c->stream));
-        HIPCHK(hipMemcpy(out, c->dp.bt.p[buf] + (int64_t)rhs * c->hp.bufsize[buf] + off,
-                         (size_t)count * sizeof(double2), hipMemcpyDeviceToHost));
+        if (c->dp.mixed) {
+            std::vector<float2> tmp((size_t)count);
+            HIPCHK(hipMemcpy(tmp.data(), (float2*)c->dp.bt.p[buf] + (int64_t)rhs * c->hp.bufsize[buf] + off,
+                             (size_t)count * sizeof(float2), hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < count; ++i) { out[2 * i] = tmp[i].x; out[2 * i + 1] = tmp[i].y; }
+        } else {
+            HIPCHK(hipMemcpy(out, (double2*)c->dp.bt.p[buf] + (int64_t)rhs * c->hp.bufsize[buf] + off,
+                             (size_t)count * sizeof(double2), hipMemcpyDeviceToHost));
+        }
     } catch (const std::exception& e) {
         return fail(DDFMM_EDEVICE, e.what());
     }
@@ -357,13 +373,15 @@ int ddfmm_set_timing(ddfmm_ctx* c, int on) {
     return DDFMM_OK;
 }
 
-int ddfmm_stage_times(ddfmm_ctx* c, int cap, char* names, double* ms, int* count) {
+int ddfmm_stage_times(ddfmm_ctx* c, int cap, char* names, double* ms, double* work, int* kinds, int* count) {
     if (!c || !count) return fail(DDFMM_EINVAL, "null argument");
     const auto& st = c->hp.stages;
     *count = (int)st.size();
     for (int i = 0; i < (int)st.size() && i < cap; ++i) {
         if (names) std::memcpy(names + 24 * i, st[i].name, 24);
         if (ms) ms[i] = i < (int)c->stage_ms.size() ? c->stage_ms[i] : 0.0;
+        if (work) work[i] = st[i].work;
+        if (kinds) kinds[i] = st[i].kind + 16 * st[i].prec;
     }
     return DDFMM_OK;
 }

@@ -8,17 +8,19 @@
 namespace ddfmm {
 
 struct BufTable {
-    double2* p[BUF_COUNT];
+    void* p[BUF_COUNT];            // double2 (FP64 mode) or float2 (mixed mode) elements
     int64_t stride[BUF_COUNT];     // elements per rhs slab
 };
 
 struct DevPlan {
     int64_t N = 0;
     int P = 0, NL = 0, ktype = 0;
+    int mixed = 0;                 // 1: FP32 vectors/translations, FP64 solves
     double *px = nullptr, *py = nullptr, *pz = nullptr;
     int* perm = nullptr;
     int* grid = nullptr;
     double2* opmat = nullptr;
+    float2* opmat32 = nullptr;     // FP32 copy of the operator table (mixed mode)
     KTask* ktasks = nullptr;
     KSrc* ksrcs = nullptr;
     GGroup* groups = nullptr;
@@ -36,7 +38,8 @@ hipError_t launch_stage(const DevPlan& d, const Stage& s, int nrhs, hipStream_t 
 hipError_t launch_kgen(const KGen* jobs, int njobs, const double* gp, double2* opmat, int ktype, hipStream_t st);
 hipError_t launch_direct(const double* trg, int M, const double* src, int Ns, const double2* den, int nrhs,
                          double2* out, int ktype, hipStream_t st);
-hipError_t launch_kmatrix(const double* trg, int M, const double* src, int Ns, double2* out, int ktype,
+hipError_t launch_kmatrix(const double* trg, int M, const double* src, int Ns, void* out, int ktype, int f32,
                           hipStream_t st);
+hipError_t launch_to_f32(const double2* in, float2* out, int64_t n, hipStream_t st);
 
 }  // namespace ddfmm

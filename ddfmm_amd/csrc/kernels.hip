@@ -1,16 +1,21 @@
 // HIP kernels of the directional-FMM evaluation for gfx950 (CDNA4, wave64).
 //
-//   kapply_kernel  generated-kernel block apply: out(t_i) += sum_j G(t_i, s_j) v_j with
+//   kapply_kernel  generated-kernel block apply: out(t_i) (+)= sum_j G(t_i, s_j) v_j with
 //                  LDS-staged source tiles; serves the near field (U list), S2M, L2T,
 //                  W/X lists (Kernel3d::kernel + zgemv, kernel3d.cpp:26-107,
 //                  wave3d_eval.cpp:441-451, 550-560, 585-718)
-//   gemm_kernel    grouped complex GEMM over (box, rhs) columns with gather/scatter and
-//                  store/add/atomic epilogues: all translations that the reference does
-//                  as per-box zgemv (vecmatop.cpp:102-125)
+//   gemm_kernel    grouped complex GEMM on MFMA (16x16x4, f64 or f32) over (box, rhs)
+//                  columns with gather/scatter and store/add/atomic epilogues: every
+//                  translation the reference applies as a per-box zgemv (vecmatop.cpp:102-125)
 //   fft_fwd_kernel / vlist_kernel   FFT-diagonal LF M2L on the (2P)^3 grid
 //                  (wave3d_eval.cpp:602-661)
 //   kgen_kernel    setup-time translation-matrix generation (mlib3d.cpp:106-280 and the
 //                  per-offset HF M2L matrices of wave3d_eval.cpp:876-928)
+//
+// Vectors are double2 (FP64 mode) or float2 (mixed mode).  In the mixed mode the
+// kernel G is evaluated from FP64 coordinate differences with FP32 sqrt/rcp and the
+// hardware sin/cos on the phase reduced to [-1/2, 1/2] revolutions (k = 2*pi, so
+// exp(ikr) = exp(2*pi*i*(r - rint(r)))).
 #include <hip/hip_runtime.h>
 
 #include "device.hpp"
@@ -19,16 +24,26 @@ namespace ddfmm {
 
 namespace {
 
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+template <class V> struct Real;
+template <> struct Real<double2> { using T = double; };
+template <> struct Real<float2> { using T = float; };
+
+template <class V>
+__device__ __forceinline__ V mk(double x, double y) {
+    V v;
+    v.x = (typename Real<V>::T)x;
+    v.y = (typename Real<V>::T)y;
+    return v;
 }
-__device__ __forceinline__ void cfma(double2& acc, double2 a, double2 b) {
+
+template <class V>
+__device__ __forceinline__ void cfma(V& acc, V a, V b) {
     acc.x = fma(a.x, b.x, fma(-a.y, b.y, acc.x));
     acc.y = fma(a.x, b.y, fma(a.y, b.x, acc.y));
 }
 
-// G(r) for k = 2*pi: e^{ikr}/r (type 0) or e^{ikr} (type 1); r^2 < 1e-16 => r := 1.
-__device__ __forceinline__ double2 green(double dx, double dy, double dz, int ktype) {
+// FP64 G(r), k = 2*pi: e^{ikr}/r (type 0) or e^{ikr} (type 1); r^2 < 1e-16 => r := 1.
+__device__ __forceinline__ double2 green64(double dx, double dy, double dz, int ktype) {
     double r2 = dx * dx + dy * dy + dz * dz;
     r2 = r2 < 1e-16 ? 1.0 : r2;
     double r = sqrt(r2);
@@ -41,9 +56,40 @@ __device__ __forceinline__ double2 green(double dx, double dy, double dz, int kt
     return make_double2(c, s);
 }
 
+// FP32 G(r) from FP64 differences: v_sqrt_f32, v_rcp_f32, v_sin_f32/v_cos_f32 on the
+// phase r - rint(r) in revolutions.
+__device__ __forceinline__ float2 green32(double dx, double dy, double dz, int ktype) {
+    float fx = (float)dx, fy = (float)dy, fz = (float)dz;
+    float r2 = fx * fx + fy * fy + fz * fz;
+    r2 = r2 < 1e-16f ? 1.0f : r2;
+    float r = __builtin_amdgcn_sqrtf(r2);
+    float t = r - __builtin_rintf(r);
+    float s = __builtin_amdgcn_sinf(t), c = __builtin_amdgcn_cosf(t);
+    if (ktype == 0) {
+        float ir = __builtin_amdgcn_rcpf(r);
+        return make_float2(c * ir, s * ir);
+    }
+    return make_float2(c, s);
+}
+
+template <class V>
+__device__ __forceinline__ V green(double dx, double dy, double dz, int ktype);
+template <>
+__device__ __forceinline__ double2 green<double2>(double dx, double dy, double dz, int ktype) {
+    return green64(dx, dy, dz, ktype);
+}
+template <>
+__device__ __forceinline__ float2 green<float2>(double dx, double dy, double dz, int ktype) {
+    return green32(dx, dy, dz, ktype);
+}
+
+template <class V>
+__device__ __forceinline__ V* bufp(const BufTable& bt, int b) { return (V*)bt.p[b]; }
+
+// ---------------------------------------------------------------- kernel apply
 constexpr int KA_THREADS = 128;
 
-template <int RC>
+template <class V, int RC>
 __global__ __launch_bounds__(KA_THREADS) void kapply_kernel(
     const KTask* __restrict__ tasks, const KSrc* __restrict__ srcs,
     const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
@@ -52,7 +98,7 @@ __global__ __launch_bounds__(KA_THREADS) void kapply_kernel(
     const int r0 = blockIdx.y * RC;
     const int nr = min(RC, nrhs - r0);
     __shared__ double sx[KA_THREADS], sy[KA_THREADS], sz[KA_THREADS];
-    __shared__ double2 sv[RC][KA_THREADS];
+    __shared__ V sv[RC][KA_THREADS];
     for (int tb = 0; tb < t.count; tb += KA_THREADS) {
         const int i = tb + threadIdx.x;
         const bool ok = i < t.count;
@@ -62,12 +108,12 @@ __global__ __launch_bounds__(KA_THREADS) void kapply_kernel(
             ty = py[t.off + i] * t.scale + t.c[1];
             tz = pz[t.off + i] * t.scale + t.c[2];
         }
-        double2 acc[RC];
+        V acc[RC];
 #pragma unroll
-        for (int q = 0; q < RC; ++q) acc[q] = make_double2(0, 0);
+        for (int q = 0; q < RC; ++q) acc[q] = mk<V>(0, 0);
         for (int si = t.s0; si < t.s1; ++si) {
             const KSrc s = srcs[si];
-            const double2* vb = bt.p[s.vbuf] + s.voff;
+            const V* vb = bufp<V>(bt, s.vbuf) + s.voff;
             const int64_t vs = bt.stride[s.vbuf];
             for (int sb = 0; sb < s.count; sb += KA_THREADS) {
                 const int n = min(KA_THREADS, s.count - sb);
@@ -79,13 +125,12 @@ __global__ __launch_bounds__(KA_THREADS) void kapply_kernel(
                     sz[threadIdx.x] = pz[j] * s.scale + s.c[2];
 #pragma unroll
                     for (int q = 0; q < RC; ++q)
-                        sv[q][threadIdx.x] = q < nr ? vb[(int64_t)(r0 + q) * vs + sb + threadIdx.x]
-                                                    : make_double2(0, 0);
+                        sv[q][threadIdx.x] = q < nr ? vb[(int64_t)(r0 + q) * vs + sb + threadIdx.x] : mk<V>(0, 0);
                 }
                 __syncthreads();
                 if (ok) {
                     for (int j = 0; j < n; ++j) {
-                        double2 g = green(tx - sx[j], ty - sy[j], tz - sz[j], ktype);
+                        const V g = green<V>(tx - sx[j], ty - sy[j], tz - sz[j], ktype);
 #pragma unroll
                         for (int q = 0; q < RC; ++q) cfma(acc[q], g, sv[q][j]);
                     }
@@ -93,179 +138,245 @@ __global__ __launch_bounds__(KA_THREADS) void kapply_kernel(
             }
         }
         if (ok) {
-            double2* ob = bt.p[t.obuf] + t.ooff + i;
+            V* ob = bufp<V>(bt, t.obuf) + t.ooff + i;
             const int64_t os = bt.stride[t.obuf];
-            for (int q = 0; q < nr; ++q) {
-                double2* o = ob + (int64_t)(r0 + q) * os;
+#pragma unroll
+            for (int q = 0; q < RC; ++q) {
+                if (q >= nr) break;
+                V* o = ob + (int64_t)(r0 + q) * os;
                 if (t.mode == 0) {
                     *o = acc[q];
                 } else {
-                    double2 v = *o;
-                    *o = make_double2(v.x + acc[q].x, v.y + acc[q].y);
+                    V v = *o;
+                    v.x += acc[q].x;
+                    v.y += acc[q].y;
+                    *o = v;
                 }
             }
         }
     }
 }
 
-// ---------------------------------------------------------------- grouped GEMM
-constexpr int GK = 16;
+// ---------------------------------------------------------------- grouped complex GEMM (MFMA)
+template <class R> struct Mfma;
+template <> struct Mfma<float> {
+    typedef float acc4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc4 op(float a, float b, acc4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Mfma<double> {
+    typedef double acc4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc4 op(double a, double b, acc4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
 
+constexpr int GK = 16;           // k-slab staged in LDS
+constexpr int LDSP = GEMM_TM + 4;  // padded LDS row
+
+template <class V>
+__device__ __forceinline__ void atomic_add(V* p, typename Real<V>::T x, typename Real<V>::T y) {
+    unsafeAtomicAdd(&p->x, x);
+    unsafeAtomicAdd(&p->y, y);
+}
+
+// C[:, ccol] (=|+=|atomic +=) diag(scale) * op(A) * B[:, bcol]; block tile 64 x 64, four
+// waves of 32 x 32, each a 2 x 2 grid of 16 x 16 MFMA tiles; complex product as four
+// real MFMA products per k-step.
+template <class TA, class V, class R>
 __global__ __launch_bounds__(256) void gemm_kernel(
     const GGroup* __restrict__ groups, const GTile* __restrict__ tiles,
     const int64_t* __restrict__ colB, const int64_t* __restrict__ colC,
-    const double2* __restrict__ opmat, BufTable bt, int tile0) {
+    const TA* __restrict__ opm, BufTable bt, int tile0) {
+    using M = Mfma<R>;
     const GTile tl = tiles[tile0 + blockIdx.x];
     const GGroup g = groups[tl.g];
     const int rhs = blockIdx.y;
     const int m0 = tl.mt * GEMM_TM, n0 = tl.nt * GEMM_TN;
-    const double2* A = opmat + g.aoff;
-    const double2* B = bt.p[g.bbuf] + (int64_t)rhs * bt.stride[g.bbuf];
-    double2* C = bt.p[g.cbuf] + (int64_t)rhs * bt.stride[g.cbuf];
-    __shared__ double2 As[GK][GEMM_TM + 1];
-    __shared__ double2 Bs[GK][GEMM_TN + 1];
+    const TA* A = opm + g.aoff;
+    const V* B = bufp<V>(bt, g.bbuf) + (int64_t)rhs * bt.stride[g.bbuf];
+    V* C = bufp<V>(bt, g.cbuf) + (int64_t)rhs * bt.stride[g.cbuf];
+    __shared__ R Ar[GK][LDSP], Ai[GK][LDSP];
+    __shared__ R Br[GK][LDSP], Bi[GK][LDSP];
     __shared__ int64_t cb[GEMM_TN];
     const int tid = threadIdx.x;
-    const int tx = tid % 16, ty = tid / 16;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
     if (tid < GEMM_TN) cb[tid] = (n0 + tid < g.ncols) ? colB[g.col0 + n0 + tid] : -1;
-    double2 acc[4][2];
+    typename M::acc4 cr[2][2], ci[2][2];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = make_double2(0, 0);
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) { cr[a][b][v] = 0; ci[a][b][v] = 0; }
     for (int k0 = 0; k0 < g.K; k0 += GK) {
         __syncthreads();
 #pragma unroll
-        for (int l = 0; l < 4; ++l) {
+        for (int l = 0; l < (GEMM_TM * GK) / 256; ++l) {
             const int e = tid + 256 * l;
             int mm, kk;
             if (g.transA) { kk = e % GK; mm = e / GK; }
             else { mm = e % GEMM_TM; kk = e / GEMM_TM; }
             const int m = m0 + mm, k = k0 + kk;
-            double2 v = make_double2(0, 0);
-            if (m < g.M && k < g.K)
-                v = g.transA ? A[k + (int64_t)m * g.lda] : A[m + (int64_t)k * g.lda];
-            As[kk][mm] = v;
+            R re = 0, im = 0;
+            if (m < g.M && k < g.K) {
+                const TA a = g.transA ? A[k + (int64_t)m * g.lda] : A[m + (int64_t)k * g.lda];
+                re = (R)a.x;
+                im = (R)a.y;
+            }
+            Ar[kk][mm] = re;
+            Ai[kk][mm] = im;
         }
 #pragma unroll
-        for (int l = 0; l < 2; ++l) {
+        for (int l = 0; l < (GEMM_TN * GK) / 256; ++l) {
             const int e = tid + 256 * l;
             const int kk = e % GK, nn = e / GK;
             const int k = k0 + kk;
             const int64_t c = cb[nn];
-            Bs[kk][nn] = (c >= 0 && k < g.K) ? B[c + k] : make_double2(0, 0);
+            R re = 0, im = 0;
+            if (c >= 0 && k < g.K) {
+                const V b = B[c + k];
+                re = (R)b.x;
+                im = (R)b.y;
+            }
+            Br[kk][nn] = re;
+            Bi[kk][nn] = im;
         }
         __syncthreads();
 #pragma unroll
-        for (int kk = 0; kk < GK; ++kk) {
-            double2 a[4], b[2];
+        for (int kk = 0; kk < GK; kk += 4) {
+            R ar[2], ai[2], br[2], bi[2];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = As[kk][tx + 16 * i];
+            for (int a = 0; a < 2; ++a) {
+                ar[a] = Ar[kk + lk][wm + 16 * a + li];
+                ai[a] = Ai[kk + lk][wm + 16 * a + li];
+            }
 #pragma unroll
-            for (int j = 0; j < 2; ++j) b[j] = Bs[kk][ty + 16 * j];
+            for (int b = 0; b < 2; ++b) {
+                br[b] = Br[kk + lk][wn + 16 * b + li];
+                bi[b] = Bi[kk + lk][wn + 16 * b + li];
+            }
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int a = 0; a < 2; ++a)
 #pragma unroll
-                for (int j = 0; j < 2; ++j) cfma(acc[i][j], a[i], b[j]);
+                for (int b = 0; b < 2; ++b) {
+                    cr[a][b] = M::op(ar[a], br[b], cr[a][b]);
+                    cr[a][b] = M::op(-ai[a], bi[b], cr[a][b]);
+                    ci[a][b] = M::op(ar[a], bi[b], ci[a][b]);
+                    ci[a][b] = M::op(ai[a], br[b], ci[a][b]);
+                }
         }
     }
+    // D layout of a 16x16 MFMA tile: lane holds rows 4*(lane/16)+v, column lane%16
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        const int n = n0 + ty + 16 * j;
+    for (int b = 0; b < 2; ++b) {
+        const int n = n0 + wn + 16 * b + li;
         if (n >= g.ncols) continue;
-        double2* Cc = C + colC[g.col0 + n];
+        V* Cc = C + colC[g.col0 + n];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int m = m0 + tx + 16 * i;
-            if (m >= g.M) continue;
-            double2 v = acc[i][j];
-            if (g.soff >= 0) v = cmul(v, opmat[g.soff + m]);
-            if (g.mode == 0) {
-                Cc[m] = v;
-            } else if (g.mode == 1) {
-                double2 o = Cc[m];
-                Cc[m] = make_double2(o.x + v.x, o.y + v.y);
-            } else {
-                unsafeAtomicAdd(&Cc[m].x, v.x);
-                unsafeAtomicAdd(&Cc[m].y, v.y);
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int m = m0 + wm + 16 * a + 4 * lk + v;
+                if (m >= g.M) continue;
+                R re = cr[a][b][v], im = ci[a][b][v];
+                if (g.soff >= 0) {
+                    const TA s = opm[g.soff + m];
+                    const R sr = (R)s.x, si = (R)s.y;
+                    const R t = re * sr - im * si;
+                    im = re * si + im * sr;
+                    re = t;
+                }
+                if (g.mode == 0) {
+                    Cc[m] = mk<V>(re, im);
+                } else if (g.mode == 1) {
+                    V o = Cc[m];
+                    o.x += (typename Real<V>::T)re;
+                    o.y += (typename Real<V>::T)im;
+                    Cc[m] = o;
+                } else {
+                    atomic_add(&Cc[m], (typename Real<V>::T)re, (typename Real<V>::T)im);
+                }
             }
-        }
     }
 }
 
 // ---------------------------------------------------------------- (2P)^3 DFT in LDS
-template <int G>
+template <int G, class V>
 struct Dft {
     // twiddles w_k = exp(-2 pi i k / G) (forward); the inverse uses their conjugates
-    __device__ static void twiddles(double2* tw) {
+    __device__ static void twiddles(V* tw) {
         if ((int)threadIdx.x < G) {
             double s, c;
             sincospi(2.0 * threadIdx.x / G, &s, &c);
-            tw[threadIdx.x] = make_double2(c, -s);
+            tw[threadIdx.x] = mk<V>(c, -s);
         }
     }
     // one pass of a length-G DFT along axis ax (0 fastest) from in to out
-    __device__ static void pass(const double2* in, double2* out, const double2* tw, int ax, bool inverse) {
+    __device__ static void pass(const V* in, V* out, const V* tw, int ax, bool inverse) {
         constexpr int G3 = G * G * G;
         const int st = ax == 0 ? 1 : (ax == 1 ? G : G * G);
-        const double sg = inverse ? -1.0 : 1.0;
         for (int e = threadIdx.x; e < G3; e += blockDim.x) {
             const int p = (e / st) % G;
             const int base = e - p * st;
-            double2 acc = make_double2(0, 0);
+            V acc = mk<V>(0, 0);
 #pragma unroll
             for (int q = 0; q < G; ++q) {
-                const double2 w = tw[(p * q) % G];
-                cfma(acc, in[base + q * st], make_double2(w.x, sg * w.y));
+                V w = tw[(p * q) % G];
+                if (inverse) w.y = -w.y;
+                cfma(acc, in[base + q * st], w);
             }
             out[e] = acc;
         }
     }
 };
 
-template <int G>
+template <int G, class V>
 __global__ __launch_bounds__(256) void fft_fwd_kernel(const FftSrc* __restrict__ fs, const int* __restrict__ grid,
                                                       int NL, BufTable bt, int f0) {
     constexpr int G3 = G * G * G;
-    __shared__ double2 g0[G3], g1[G3];
-    __shared__ double2 tw[G];
-    Dft<G>::twiddles(tw);
+    __shared__ V g0[G3], g1[G3];
+    __shared__ V tw[G];
+    Dft<G, V>::twiddles(tw);
     const FftSrc f = fs[f0 + blockIdx.x];
     const int rhs = blockIdx.y;
-    const double2* in = bt.p[BUF_LF_UPEQ] + (int64_t)rhs * bt.stride[BUF_LF_UPEQ] + f.in_off;
-    double2* out = bt.p[BUF_SPEC] + (int64_t)rhs * bt.stride[BUF_SPEC] + f.spec_off;
-    for (int e = threadIdx.x; e < G3; e += blockDim.x) g0[e] = make_double2(0, 0);
+    const V* in = bufp<V>(bt, BUF_LF_UPEQ) + (int64_t)rhs * bt.stride[BUF_LF_UPEQ] + f.in_off;
+    V* out = bufp<V>(bt, BUF_SPEC) + (int64_t)rhs * bt.stride[BUF_SPEC] + f.spec_off;
+    for (int e = threadIdx.x; e < G3; e += blockDim.x) g0[e] = mk<V>(0, 0);
     __syncthreads();
     for (int k = threadIdx.x; k < NL; k += blockDim.x) g0[grid[f.grid + k]] = in[k];
     __syncthreads();
-    Dft<G>::pass(g0, g1, tw, 0, false);
+    Dft<G, V>::pass(g0, g1, tw, 0, false);
     __syncthreads();
-    Dft<G>::pass(g1, g0, tw, 1, false);
+    Dft<G, V>::pass(g1, g0, tw, 1, false);
     __syncthreads();
-    Dft<G>::pass(g0, g1, tw, 2, false);
+    Dft<G, V>::pass(g0, g1, tw, 2, false);
     __syncthreads();
     for (int e = threadIdx.x; e < G3; e += blockDim.x) out[e] = g1[e];
 }
 
-template <int G>
+template <int G, class V>
 __global__ __launch_bounds__(256) void vlist_kernel(const VTarget* __restrict__ vt, const VPair* __restrict__ vp,
-                                                    const int* __restrict__ grid, const double2* __restrict__ opmat,
+                                                    const int* __restrict__ grid, const V* __restrict__ opm,
                                                     int NL, BufTable bt, int v0) {
     constexpr int G3 = G * G * G;
     constexpr int PER = (G3 + 255) / 256;
-    __shared__ double2 g0[G3], g1[G3];
-    __shared__ double2 tw[G];
-    Dft<G>::twiddles(tw);
+    __shared__ V g0[G3], g1[G3];
+    __shared__ V tw[G];
+    Dft<G, V>::twiddles(tw);
     const VTarget t = vt[v0 + blockIdx.x];
     const int rhs = blockIdx.y;
-    const double2* spec = bt.p[BUF_SPEC] + (int64_t)rhs * bt.stride[BUF_SPEC];
-    double2 acc[PER];
+    const V* spec = bufp<V>(bt, BUF_SPEC) + (int64_t)rhs * bt.stride[BUF_SPEC];
+    V acc[PER];
 #pragma unroll
-    for (int l = 0; l < PER; ++l) acc[l] = make_double2(0, 0);
+    for (int l = 0; l < PER; ++l) acc[l] = mk<V>(0, 0);
     for (int p = t.p0; p < t.p1; ++p) {
         const VPair pr = vp[p];
-        const double2* s = spec + pr.spec_off;
-        const double2* T = opmat + t.tab_off + (int64_t)pr.offidx * G3;
+        const V* s = spec + pr.spec_off;
+        const V* T = opm + t.tab_off + (int64_t)pr.offidx * G3;
 #pragma unroll
         for (int l = 0; l < PER; ++l) {
             const int e = threadIdx.x + 256 * l;
@@ -278,18 +389,20 @@ __global__ __launch_bounds__(256) void vlist_kernel(const VTarget* __restrict__ 
         if (e < G3) g0[e] = acc[l];
     }
     __syncthreads();
-    Dft<G>::pass(g0, g1, tw, 0, true);
+    Dft<G, V>::pass(g0, g1, tw, 0, true);
     __syncthreads();
-    Dft<G>::pass(g1, g0, tw, 1, true);
+    Dft<G, V>::pass(g1, g0, tw, 1, true);
     __syncthreads();
-    Dft<G>::pass(g0, g1, tw, 2, true);
+    Dft<G, V>::pass(g0, g1, tw, 2, true);
     __syncthreads();
-    double2* out = bt.p[BUF_LF_CHK] + (int64_t)rhs * bt.stride[BUF_LF_CHK] + t.out_off;
-    const double coef = 1.0 / G3;
+    V* out = bufp<V>(bt, BUF_LF_CHK) + (int64_t)rhs * bt.stride[BUF_LF_CHK] + t.out_off;
+    const typename Real<V>::T coef = (typename Real<V>::T)(1.0 / G3);
     for (int k = threadIdx.x; k < NL; k += blockDim.x) {
-        const double2 v = g1[grid[t.grid + k]];
-        double2 o = out[k];
-        out[k] = make_double2(o.x + v.x * coef, o.y + v.y * coef);
+        const V v = g1[grid[t.grid + k]];
+        V o = out[k];
+        o.x += v.x * coef;
+        o.y += v.y * coef;
+        out[k] = o;
     }
 }
 
@@ -305,28 +418,38 @@ __global__ __launch_bounds__(256) void kgen_kernel(const KGen* __restrict__ jobs
         const double* s = gp + 3 * (k.soff + j);
         double d[3];
         for (int q = 0; q < 3; ++q) d[q] = (sg * t[q] + k.tshift[q]) - (s[q] + k.sshift[q]);
-        opmat[k.out + e] = green(d[0], d[1], d[2], ktype);
+        opmat[k.out + e] = green64(d[0], d[1], d[2], ktype);
     }
 }
 
-__global__ void zero_kernel(double2* p, int64_t n) {
+__global__ void to_f32_kernel(const double2* __restrict__ in, float2* __restrict__ out, int64_t n) {
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
-        p[e] = make_double2(0, 0);
+        out[e] = make_float2((float)in[e].x, (float)in[e].y);
 }
 
-__global__ void gather_kernel(const double2* __restrict__ in, double2* __restrict__ out, const int* __restrict__ perm,
+template <class V>
+__global__ void zero_kernel(V* p, int64_t n) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        p[e] = mk<V>(0, 0);
+}
+
+template <class V>
+__global__ void gather_kernel(const double2* __restrict__ in, V* __restrict__ out, const int* __restrict__ perm,
                               int64_t N, int nrhs) {
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < N * nrhs; e += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = e / N, i = e % N;
-        out[e] = in[r * N + perm[i]];
+        const double2 v = in[r * N + perm[i]];
+        out[e] = mk<V>(v.x, v.y);
     }
 }
 
-__global__ void scatter_kernel(const double2* __restrict__ in, double2* __restrict__ out, const int* __restrict__ perm,
+template <class V>
+__global__ void scatter_kernel(const V* __restrict__ in, double2* __restrict__ out, const int* __restrict__ perm,
                                int64_t N, int nrhs) {
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < N * nrhs; e += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = e / N, i = e % N;
-        out[r * N + perm[i]] = in[e];
+        const V v = in[e];
+        out[r * N + perm[i]] = make_double2(v.x, v.y);
     }
 }
 
@@ -355,18 +478,20 @@ __global__ __launch_bounds__(256) void direct_kernel(const double* __restrict__ 
         __syncthreads();
         if (ok)
             for (int j = 0; j < n; ++j)
-                cfma(acc, green(tx - s3[3 * j], ty - s3[3 * j + 1], tz - s3[3 * j + 2], ktype), sd[j]);
+                cfma(acc, green64(tx - s3[3 * j], ty - s3[3 * j + 1], tz - s3[3 * j + 2], ktype), sd[j]);
     }
     if (ok) out[(int64_t)rhs * M + i] = acc;
 }
 
 // dense Kernel3d::kernel block: out(i,j) = G(trg_i, src_j), column-major M x N
+template <class V>
 __global__ void kmatrix_kernel(const double* __restrict__ trg, int M, const double* __restrict__ src, int Ns,
-                               double2* __restrict__ out, int ktype) {
+                               V* __restrict__ out, int ktype) {
     const int64_t mn = (int64_t)M * Ns;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < mn; e += (int64_t)gridDim.x * blockDim.x) {
         const int i = (int)(e % M), j = (int)(e / M);
-        out[e] = green(trg[3 * i] - src[3 * j], trg[3 * i + 1] - src[3 * j + 1], trg[3 * i + 2] - src[3 * j + 2], ktype);
+        out[e] = green<V>(trg[3 * i] - src[3 * j], trg[3 * i + 1] - src[3 * j + 1], trg[3 * i + 2] - src[3 * j + 2],
+                          ktype);
     }
 }
 
@@ -375,47 +500,68 @@ int grid_for(int64_t n, int threads) {
     return (int)std::min<int64_t>(std::max<int64_t>(b, 1), 65536);
 }
 
-}  // namespace
-
-// ---------------------------------------------------------------- launchers
-hipError_t launch_stage(const DevPlan& d, const Stage& s, int nrhs, hipStream_t st) {
+template <class V>
+hipError_t launch_stage_t(const DevPlan& d, const Stage& s, int nrhs, hipStream_t st) {
     const int n = s.end - s.begin;
     switch (s.kind) {
     case ST_ZERO: {
         int64_t cnt = s.count * nrhs;   // rhs slabs are contiguous with stride = count
-        if (cnt > 0) zero_kernel<<<grid_for(cnt, 256), 256, 0, st>>>(d.bt.p[s.buf], cnt);
+        if (cnt > 0) zero_kernel<V><<<grid_for(cnt, 256), 256, 0, st>>>((V*)d.bt.p[s.buf], cnt);
         break;
     }
     case ST_GATHER:
-        gather_kernel<<<grid_for(d.N * nrhs, 256), 256, 0, st>>>(d.den_io, d.bt.p[BUF_DEN], d.perm, d.N, nrhs);
+        gather_kernel<V><<<grid_for(d.N * nrhs, 256), 256, 0, st>>>(d.den_io, (V*)d.bt.p[BUF_DEN], d.perm, d.N, nrhs);
         break;
     case ST_SCATTER:
-        scatter_kernel<<<grid_for(d.N * nrhs, 256), 256, 0, st>>>(d.bt.p[BUF_VAL], d.val_io, d.perm, d.N, nrhs);
+        scatter_kernel<V><<<grid_for(d.N * nrhs, 256), 256, 0, st>>>((const V*)d.bt.p[BUF_VAL], d.val_io, d.perm, d.N,
+                                                                     nrhs);
         break;
     case ST_KAPPLY:
         if (nrhs == 1) {
-            kapply_kernel<1><<<dim3(n, 1), KA_THREADS, 0, st>>>(d.ktasks, d.ksrcs, d.px, d.py, d.pz, d.bt,
-                                                                s.begin, nrhs, d.ktype);
+            kapply_kernel<V, 1><<<dim3(n, 1), KA_THREADS, 0, st>>>(d.ktasks, d.ksrcs, d.px, d.py, d.pz, d.bt,
+                                                                   s.begin, nrhs, d.ktype);
         } else {
-            kapply_kernel<4><<<dim3(n, (nrhs + 3) / 4), KA_THREADS, 0, st>>>(d.ktasks, d.ksrcs, d.px, d.py, d.pz,
-                                                                            d.bt, s.begin, nrhs, d.ktype);
+            kapply_kernel<V, 4><<<dim3(n, (nrhs + 3) / 4), KA_THREADS, 0, st>>>(d.ktasks, d.ksrcs, d.px, d.py, d.pz,
+                                                                               d.bt, s.begin, nrhs, d.ktype);
         }
         break;
     case ST_GEMM:
-        gemm_kernel<<<dim3(n, nrhs), 256, 0, st>>>(d.groups, d.tiles, d.colB, d.colC, d.opmat, d.bt, s.begin);
+        if (d.mixed) {
+            if (s.prec == PREC_SOLVE)
+                gemm_kernel<double2, float2, double><<<dim3(n, nrhs), 256, 0, st>>>(
+                    d.groups, d.tiles, d.colB, d.colC, d.opmat, d.bt, s.begin);
+            else
+                gemm_kernel<float2, float2, float><<<dim3(n, nrhs), 256, 0, st>>>(
+                    d.groups, d.tiles, d.colB, d.colC, d.opmat32, d.bt, s.begin);
+        } else {
+            gemm_kernel<double2, double2, double><<<dim3(n, nrhs), 256, 0, st>>>(
+                d.groups, d.tiles, d.colB, d.colC, d.opmat, d.bt, s.begin);
+        }
         break;
     case ST_FFTFWD:
         if (d.P != 6) return hipErrorInvalidValue;
-        fft_fwd_kernel<12><<<dim3(n, nrhs), 256, 0, st>>>(d.fsrc, d.grid, d.NL, d.bt, s.begin);
+        fft_fwd_kernel<12, V><<<dim3(n, nrhs), 256, 0, st>>>(d.fsrc, d.grid, d.NL, d.bt, s.begin);
         break;
     case ST_VLIST:
         if (d.P != 6) return hipErrorInvalidValue;
-        vlist_kernel<12><<<dim3(n, nrhs), 256, 0, st>>>(d.vtrg, d.vpair, d.grid, d.opmat, d.NL, d.bt, s.begin);
+        if (d.mixed)
+            vlist_kernel<12, float2><<<dim3(n, nrhs), 256, 0, st>>>(d.vtrg, d.vpair, d.grid, d.opmat32, d.NL, d.bt,
+                                                                    s.begin);
+        else
+            vlist_kernel<12, double2><<<dim3(n, nrhs), 256, 0, st>>>(d.vtrg, d.vpair, d.grid, d.opmat, d.NL, d.bt,
+                                                                     s.begin);
         break;
     default:
         return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- launchers
+hipError_t launch_stage(const DevPlan& d, const Stage& s, int nrhs, hipStream_t st) {
+    return d.mixed ? launch_stage_t<float2>(d, s, nrhs, st) : launch_stage_t<double2>(d, s, nrhs, st);
 }
 
 hipError_t launch_kgen(const KGen* jobs, int njobs, const double* gp, double2* opmat, int ktype, hipStream_t st) {
@@ -426,15 +572,23 @@ hipError_t launch_kgen(const KGen* jobs, int njobs, const double* gp, double2* o
     return hipGetLastError();
 }
 
+hipError_t launch_to_f32(const double2* in, float2* out, int64_t n, hipStream_t st) {
+    if (n > 0) to_f32_kernel<<<grid_for(n, 256), 256, 0, st>>>(in, out, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_direct(const double* trg, int M, const double* src, int Ns, const double2* den, int nrhs,
                          double2* out, int ktype, hipStream_t st) {
     direct_kernel<<<dim3((M + 255) / 256, nrhs), 256, 0, st>>>(trg, M, src, Ns, den, nrhs, out, ktype);
     return hipGetLastError();
 }
 
-hipError_t launch_kmatrix(const double* trg, int M, const double* src, int Ns, double2* out, int ktype,
+hipError_t launch_kmatrix(const double* trg, int M, const double* src, int Ns, void* out, int ktype, int f32,
                           hipStream_t st) {
-    kmatrix_kernel<<<grid_for((int64_t)M * Ns, 256), 256, 0, st>>>(trg, M, src, Ns, out, ktype);
+    if (f32)
+        kmatrix_kernel<float2><<<grid_for((int64_t)M * Ns, 256), 256, 0, st>>>(trg, M, src, Ns, (float2*)out, ktype);
+    else
+        kmatrix_kernel<double2><<<grid_for((int64_t)M * Ns, 256), 256, 0, st>>>(trg, M, src, Ns, (double2*)out, ktype);
     return hipGetLastError();
 }
 

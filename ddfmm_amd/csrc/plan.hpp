@@ -60,7 +60,7 @@ struct GTile {
     int g, mt, nt, pad;
 };
 constexpr int GEMM_TM = 64;
-constexpr int GEMM_TN = 32;
+constexpr int GEMM_TN = 64;
 
 // LF FFT-diagonal M2L (wave3d_eval.cpp:602-661)
 struct FftSrc {
@@ -94,11 +94,19 @@ struct KGen {
 enum StageKind : int {
     ST_ZERO = 0, ST_GATHER, ST_SCATTER, ST_KAPPLY, ST_GEMM, ST_FFTFWD, ST_VLIST
 };
+// Arithmetic class of a stage.  In the mixed-precision mode translation stages
+// (kernel-generated matrices: M2M/L2L, U2U/D2D child kernels, HF M2L, near field, V list)
+// run in FP32 on FP32 vectors, while the ill-conditioned check->equivalent solves
+// (LF pinv, HF E1/E2/E3 chains) keep FP64 matrices and FP64 accumulation.
+enum StagePrec : int { PREC_TRANSLATION = 0, PREC_SOLVE = 1 };
 struct Stage {
     int kind;
     int begin, end;     // descriptor range (tasks / tiles / sources / targets)
     int buf;            // ST_ZERO: buffer id
     int64_t count;      // ST_ZERO: elements per rhs
+    int prec;           // StagePrec
+    int pad;
+    double work;        // per RHS: GEMM useful flops, KAPPLY interactions, FFT spectra, V-list pairs
     char name[24];
 };
 

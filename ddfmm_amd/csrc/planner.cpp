@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cstring>
 #include <sstream>
+#include <string>
 
 namespace ddfmm {
 
@@ -84,6 +85,11 @@ struct Builder {
         std::memset(&s, 0, sizeof(s));
         s.kind = kind; s.begin = b; s.end = e; s.buf = buf; s.count = count;
         std::strncpy(s.name, name, sizeof(s.name) - 1);
+        // solves: LF pinv factors and HF E1/E2/E3 chains
+        const std::string nm(name);
+        s.prec = (nm.find("uc2ue") != std::string::npos || nm.find("dc2de") != std::string::npos ||
+                  nm.find("_e1") != std::string::npos || nm.find("_e2") != std::string::npos ||
+                  nm.find("_e3") != std::string::npos) ? PREC_SOLVE : PREC_TRANSLATION;
         pl.stages.push_back(s);
     }
     int64_t gen(int64_t toff, int m, const double ts[3], int tneg, int64_t soff, int n,
@@ -684,6 +690,31 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
     }
     stage(ST_KAPPLY, t0, (int)pl.ktasks.size(), "leaf_near_l2t");
     stage(ST_SCATTER, 0, 0, "scatter");
+
+    // ---------------- per-stage algorithmic work (per RHS)
+    for (auto& s : pl.stages) {
+        double w = 0;
+        if (s.kind == ST_GEMM) {
+            int last = -1;
+            for (int t = s.begin; t < s.end; ++t) {
+                int gi = pl.tiles[t].g;
+                if (gi == last) continue;
+                last = gi;
+                const GGroup& g = pl.groups[gi];
+                w += 8.0 * g.M * g.K * g.ncols;
+            }
+        } else if (s.kind == ST_KAPPLY) {
+            for (int t = s.begin; t < s.end; ++t) {
+                const KTask& k = pl.ktasks[t];
+                for (int q = k.s0; q < k.s1; ++q) w += (double)k.count * pl.ksrcs[q].count;
+            }
+        } else if (s.kind == ST_FFTFWD) {
+            w = s.end - s.begin;
+        } else if (s.kind == ST_VLIST) {
+            for (int t = s.begin; t < s.end; ++t) w += pl.vtrg[t].p1 - pl.vtrg[t].p0;
+        }
+        s.work = w;
+    }
 
     // ---------------- canonical dump of the lists (integer parity with the oracle)
     if (want_dump) {

@@ -250,7 +250,7 @@ def library_paths(root: str, ktype: int, accu: int, NPQ: int):
 
 
 def ensure_library(root: str, ktype: int = 0, accu: int = 2, NPQ: int = 4,
-                   hf_max_W: int = 4, lf_min_W: float = 2.0 ** -5, seed: int = 0,
+                   hf_max_W: int = 4, lf_min_W: float = 2.0 ** -3, seed: int = 0,
                    workers: int | None = None, verbose: bool = False):
     """Generate (or extend) the LF/HF operator files under ``root``; returns paths.
 

@@ -163,12 +163,17 @@ class Wave3d:
         check(lib().ddfmm_set_timing(self._ctx, int(on)))
 
     def stage_times(self):
+        """[(name, ms, work_per_rhs, kind, precision_class)] of the last timed eval."""
         cnt = ctypes.c_int()
-        check(lib().ddfmm_stage_times(self._ctx, 0, None, None, ctypes.byref(cnt)))
-        names = ctypes.create_string_buffer(24 * cnt.value)
-        ms = (ctypes.c_double * cnt.value)()
-        check(lib().ddfmm_stage_times(self._ctx, cnt.value, names, ms, ctypes.byref(cnt)))
-        return [(names.raw[24 * i:24 * i + 24].split(b"\0")[0].decode(), ms[i]) for i in range(cnt.value)]
+        check(lib().ddfmm_stage_times(self._ctx, 0, None, None, None, None, ctypes.byref(cnt)))
+        n = cnt.value
+        names = ctypes.create_string_buffer(24 * n)
+        ms = (ctypes.c_double * n)()
+        work = (ctypes.c_double * n)()
+        kinds = (ctypes.c_int * n)()
+        check(lib().ddfmm_stage_times(self._ctx, n, names, ms, work, kinds, ctypes.byref(cnt)))
+        return [(names.raw[24 * i:24 * i + 24].split(b"\0")[0].decode(), ms[i], work[i], kinds[i] % 16,
+                 kinds[i] // 16) for i in range(n)]
 
     def close(self):
         if self._ctx is not None:

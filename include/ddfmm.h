@@ -84,9 +84,13 @@ int ddfmm_dump_lists(ddfmm_ctx* ctx, char* buf, int64_t cap, int64_t* len);
    densities/potentials, LF/HF equivalent densities and check values; numbering of
    ddfmm_amd/csrc/plan.hpp) for right-hand side `rhs`, starting at `off`. */
 int ddfmm_read_buffer(ddfmm_ctx* ctx, int buf, int rhs, int64_t off, int64_t count, double* out);
-/* Per-stage timing: enable, then read names and milliseconds of the last eval. */
+/* Per-stage timing: enable, then read, per stage of the last eval, its name, milliseconds
+   (HIP events on the evaluation stream), algorithmic work per RHS (GEMM: useful complex
+   flops 8*M*K*N; kernel apply: target-source interactions; FFT: spectra; V list: pairs)
+   and kind (stage kind + 16 * arithmetic class, 0 translation / 1 FP64 solve). */
 int ddfmm_set_timing(ddfmm_ctx* ctx, int on);
-int ddfmm_stage_times(ddfmm_ctx* ctx, int cap, char* names /* cap x 24 */, double* ms, int* count);
+int ddfmm_stage_times(ddfmm_ctx* ctx, int cap, char* names /* cap x 24 */, double* ms, double* work,
+                      int* kinds, int* count);
 
 #ifdef __cplusplus
 }

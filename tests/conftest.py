@@ -16,13 +16,13 @@ def pytest_configure(config):
 def oplib():
     """(ldpath, hdpath) of the e^{ikr}/r operator library (ACCU=2, NPQ=4)."""
     from ddfmm_amd import mlib_gen
-    return mlib_gen.ensure_library(OPS, ktype=0, accu=2, NPQ=4, hf_max_W=4, lf_min_W=2.0 ** -5)
+    return mlib_gen.ensure_library(OPS, ktype=0, accu=2, NPQ=4, hf_max_W=4, lf_min_W=2.0 ** -3)
 
 
 @pytest.fixture(scope="session")
 def oplib_expr():
     from ddfmm_amd import mlib_gen
-    return mlib_gen.ensure_library(OPS, ktype=1, accu=2, NPQ=4, hf_max_W=4, lf_min_W=2.0 ** -5)
+    return mlib_gen.ensure_library(OPS, ktype=1, accu=2, NPQ=4, hf_max_W=4, lf_min_W=2.0 ** -3)
 
 
 @pytest.fixture(scope="session")

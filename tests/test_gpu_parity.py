@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from ddfmm_amd import geometry
-from ddfmm_amd.wave3d import Kernel3d, Mlib3d, Wave3d, direct_sum
+from ddfmm_amd.wave3d import FP64, MIXED, Kernel3d, Mlib3d, Wave3d, direct_sum
 from oracle import wave3d_ref as R
 
 pytestmark = pytest.mark.gpu
@@ -17,7 +17,7 @@ FP64_TOL = 1e-9
 FMM_TOL = 1e-5
 
 
-def _wave(pos, K, oplib, kernel=0, nrhs_max=1, ptsmax=100):
+def _wave(pos, K, oplib, kernel=0, nrhs_max=1, ptsmax=100, precision=FP64):
     ml = Mlib3d()
     ml.setup({"-mlib3d_ldname": oplib[0], "-mlib3d_hdname": oplib[1]})
     w = Wave3d()
@@ -25,7 +25,7 @@ def _wave(pos, K, oplib, kernel=0, nrhs_max=1, ptsmax=100):
     w.mlibptr = ml
     w.kernel = Kernel3d(kernel)
     w.setup({"-wave3d_ACCU": 2, "-wave3d_K": K, "-wave3d_maxlevel": 12, "-wave3d_ptsmax": ptsmax},
-            nrhs_max=nrhs_max)
+            nrhs_max=nrhs_max, precision=precision)
     return w
 
 
@@ -148,3 +148,39 @@ def test_eval_device_buffers(oplib):
     hip.free(v)
     assert rel(got, host) < 1e-12
     w.close()
+
+
+MIXED_VS_FP64_TOL = 1e-6   # FP32 translations/near field; FP64 solves (predicted ~1e-7)
+
+
+def test_mixed_precision_sphere60k(oplib):
+    K, N = 8.0, 60000
+    pos = geometry.sphere(N, K, 0)
+    den = geometry.densities(N, 0)
+    w = _wave(pos, K, oplib)
+    v64 = w.eval(den)
+    w.close()
+    wm = _wave(pos, K, oplib, precision=MIXED)
+    vm = wm.eval(den)
+    chk = geometry.check_targets(N, 1000, 0)
+    e_direct = rel(vm[chk], R.direct_sum(pos[chk], pos, den))
+    e64 = rel(vm, v64)
+    print(f"mixed sphere60k: vs fp64 FMM {e64:.3e}, vs direct {e_direct:.3e}", flush=True)
+    assert e64 < MIXED_VS_FP64_TOL
+    assert e_direct < FMM_TOL
+    wm.close()
+
+
+def test_mixed_multi_rhs_and_expr(oplib_expr):
+    K, N = 4.0, 5000
+    pos = geometry.sphere(N, K, 6)
+    den = geometry.densities(N, 6, nrhs=6)
+    w = _wave(pos, K, oplib_expr, kernel=1, nrhs_max=6)
+    v64 = w.eval(den)
+    w.close()
+    wm = _wave(pos, K, oplib_expr, kernel=1, nrhs_max=6, precision=MIXED)
+    vm = wm.eval(den)
+    assert rel(vm, v64) < MIXED_VS_FP64_TOL
+    one = wm.eval(den[4])
+    assert rel(one, vm[4]) < 1e-6       # FP32 atomics reorder sums
+    wm.close()

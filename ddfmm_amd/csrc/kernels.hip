@@ -159,17 +159,22 @@ __global__ __launch_bounds__(KA_THREADS) void kapply_kernel(
 
 // ---------------------------------------------------------------- grouped complex GEMM (MFMA)
 template <class R> struct Mfma;
+// Operand layouts (both): A lane l holds A[l%16][l/16], B lane l holds B[l/16][l%16].
+// Output register v of lane l holds D[row(l/16, v)][l%16]; the row mapping differs
+// between the f32 and the f64 instruction (checked on gfx950).
 template <> struct Mfma<float> {
     typedef float acc4 __attribute__((ext_vector_type(4)));
     static __device__ __forceinline__ acc4 op(float a, float b, acc4 c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
 };
 template <> struct Mfma<double> {
     typedef double acc4 __attribute__((ext_vector_type(4)));
     static __device__ __forceinline__ acc4 op(double a, double b, acc4 c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lk, int v) { return lk + 4 * v; }
 };
 
 constexpr int GK = 16;           // k-slab staged in LDS
@@ -270,7 +275,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(
                 }
         }
     }
-    // D layout of a 16x16 MFMA tile: lane holds rows 4*(lane/16)+v, column lane%16
+    // D layout of a 16x16 MFMA tile: lane holds rows M::row(lane/16, v), column lane%16
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
         const int n = n0 + wn + 16 * b + li;
@@ -280,7 +285,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(
         for (int a = 0; a < 2; ++a)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int m = m0 + wm + 16 * a + 4 * lk + v;
+                const int m = m0 + wm + 16 * a + M::row(lk, v);
                 if (m >= g.M) continue;
                 R re = cr[a][b][v], im = ci[a][b][v];
                 if (g.soff >= 0) {

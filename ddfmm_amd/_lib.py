@@ -94,7 +94,20 @@ class Hip:
         self.rt.hipFree.argtypes = [vp]
         self.rt.hipMemcpy.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int]
         self.rt.hipDeviceSynchronize.argtypes = []
-        for f in ("hipMalloc", "hipFree", "hipMemcpy", "hipDeviceSynchronize"):
+        self.rt.hipMemcpyAsync.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int, vp]
+        self.rt.hipHostMalloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t, ctypes.c_uint]
+        self.rt.hipHostFree.argtypes = [vp]
+        self.rt.hipStreamCreate.argtypes = [ctypes.POINTER(vp)]
+        self.rt.hipStreamDestroy.argtypes = [vp]
+        self.rt.hipStreamSynchronize.argtypes = [vp]
+        self.rt.hipEventCreate.argtypes = [ctypes.POINTER(vp)]
+        self.rt.hipEventDestroy.argtypes = [vp]
+        self.rt.hipEventRecord.argtypes = [vp, vp]
+        self.rt.hipEventQuery.argtypes = [vp]
+        self.rt.hipEventSynchronize.argtypes = [vp]
+        for f in ("hipMalloc", "hipFree", "hipMemcpy", "hipDeviceSynchronize", "hipMemcpyAsync", "hipHostMalloc",
+                  "hipHostFree", "hipStreamCreate", "hipStreamDestroy", "hipStreamSynchronize", "hipEventCreate",
+                  "hipEventDestroy", "hipEventRecord", "hipEventQuery", "hipEventSynchronize"):
             getattr(self.rt, f).restype = ctypes.c_int
 
     def _ok(self, rc, what):
@@ -124,3 +137,46 @@ class Hip:
 
     def sync(self):
         self._ok(self.rt.hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+    # streams, events and pinned host memory (serving loop)
+    def stream(self) -> int:
+        s = ctypes.c_void_p()
+        self._ok(self.rt.hipStreamCreate(ctypes.byref(s)), "hipStreamCreate")
+        return s.value
+
+    def stream_destroy(self, s: int):
+        self._ok(self.rt.hipStreamDestroy(s), "hipStreamDestroy")
+
+    def stream_sync(self, s: int):
+        self._ok(self.rt.hipStreamSynchronize(s), "hipStreamSynchronize")
+
+    def event(self) -> int:
+        e = ctypes.c_void_p()
+        self._ok(self.rt.hipEventCreate(ctypes.byref(e)), "hipEventCreate")
+        return e.value
+
+    def event_destroy(self, e: int):
+        self._ok(self.rt.hipEventDestroy(e), "hipEventDestroy")
+
+    def record(self, e: int, s: int):
+        self._ok(self.rt.hipEventRecord(e, s), "hipEventRecord")
+
+    def done(self, e: int) -> bool:
+        rc = self.rt.hipEventQuery(e)
+        if rc == 0:
+            return True
+        if rc == 600:          # hipErrorNotReady
+            return False
+        raise DdfmmError(f"hipEventQuery failed with HIP error {rc}")
+
+    def pinned(self, nbytes: int) -> int:
+        p = ctypes.c_void_p()
+        self._ok(self.rt.hipHostMalloc(ctypes.byref(p), nbytes, 0), "hipHostMalloc")
+        return p.value
+
+    def pinned_free(self, p: int):
+        self._ok(self.rt.hipHostFree(p), "hipHostFree")
+
+    def copy_async(self, dst: int, src: int, nbytes: int, kind: int, s: int):
+        """kind: 1 host->device, 2 device->host, 3 device->device."""
+        self._ok(self.rt.hipMemcpyAsync(dst, src, nbytes, kind, s), "hipMemcpyAsync")

@@ -160,7 +160,7 @@ int ddfmm_setup(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_params* 
             HIPCHK(hipMemcpy(d.opmat, h.opmat.data(), h.opmat.size() * sizeof(double2), hipMemcpyHostToDevice));
         d.ktasks = c->upload(h.ktasks); d.ksrcs = c->upload(h.ksrcs);
         d.groups = c->upload(h.groups); d.tiles = c->upload(h.tiles);
-        d.colB = c->upload(h.colB); d.colC = c->upload(h.colC);
+        d.colB = c->upload(h.colB); d.colC = c->upload(h.colC); d.colA = c->upload(h.colA);
         d.fsrc = c->upload(h.fsrc); d.vtrg = c->upload(h.vtrg); d.vpair = c->upload(h.vpair);
         for (int b = 0; b < BUF_COUNT; ++b) {
             d.bt.stride[b] = h.bufsize[b];

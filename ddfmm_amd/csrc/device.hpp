@@ -25,7 +25,7 @@ struct DevPlan {
     KSrc* ksrcs = nullptr;
     GGroup* groups = nullptr;
     GTile* tiles = nullptr;
-    int64_t *colB = nullptr, *colC = nullptr;
+    int64_t *colB = nullptr, *colC = nullptr, *colA = nullptr;
     FftSrc* fsrc = nullptr;
     VTarget* vtrg = nullptr;
     VPair* vpair = nullptr;

@@ -192,14 +192,14 @@ __device__ __forceinline__ void atomic_add(V* p, typename Real<V>::T x, typename
 template <class TA, class V, class R>
 __global__ __launch_bounds__(256) void gemm_kernel(
     const GGroup* __restrict__ groups, const GTile* __restrict__ tiles,
-    const int64_t* __restrict__ colB, const int64_t* __restrict__ colC,
+    const int64_t* __restrict__ colB, const int64_t* __restrict__ colC, const int64_t* __restrict__ colA,
     const TA* __restrict__ opm, BufTable bt, int tile0) {
     using M = Mfma<R>;
     const GTile tl = tiles[tile0 + blockIdx.x];
     const GGroup g = groups[tl.g];
     const int rhs = blockIdx.y;
     const int m0 = tl.mt * GEMM_TM, n0 = tl.nt * GEMM_TN;
-    const TA* A = opm + g.aoff;
+    const TA* A = g.kb ? opm : opm + g.aoff;
     const V* B = bufp<V>(bt, g.bbuf) + (int64_t)rhs * bt.stride[g.bbuf];
     V* C = bufp<V>(bt, g.cbuf) + (int64_t)rhs * bt.stride[g.cbuf];
     __shared__ R Ar[GK][LDSP], Ai[GK][LDSP];
@@ -209,7 +209,10 @@ __global__ __launch_bounds__(256) void gemm_kernel(
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
     const int li = lane & 15, lk = lane >> 4;
-    if (tid < GEMM_TN) cb[tid] = (n0 + tid < g.ncols) ? colB[g.col0 + n0 + tid] : -1;
+    // contiguous B: cached column offsets; blocked B: validity flag only
+    if (tid < GEMM_TN)
+        cb[tid] = (n0 + tid < g.ncols) ? (g.kb ? 0 : colB[g.bcol0 + n0 + tid]) : -1;
+    const int nkb = g.kb ? (g.K + g.kb - 1) / g.kb : 0;
     typename M::acc4 cr[2][2], ci[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -228,7 +231,14 @@ __global__ __launch_bounds__(256) void gemm_kernel(
             const int m = m0 + mm, k = k0 + kk;
             R re = 0, im = 0;
             if (m < g.M && k < g.K) {
-                const TA a = g.transA ? A[k + (int64_t)m * g.lda] : A[m + (int64_t)k * g.lda];
+                const TA* Ab = A;
+                int kl = k;
+                if (g.kb) {                 // A column block of this k
+                    const int blk = k / g.kb;
+                    Ab = opm + colA[g.acol0 + blk];
+                    kl = k - blk * g.kb;
+                }
+                const TA a = g.transA ? Ab[kl + (int64_t)m * g.lda] : Ab[m + (int64_t)kl * g.lda];
                 re = (R)a.x;
                 im = (R)a.y;
             }
@@ -243,9 +253,17 @@ __global__ __launch_bounds__(256) void gemm_kernel(
             const int64_t c = cb[nn];
             R re = 0, im = 0;
             if (c >= 0 && k < g.K) {
-                const V b = B[c + k];
-                re = (R)b.x;
-                im = (R)b.y;
+                int64_t off = c + k;
+                if (g.kb) {
+                    const int blk = k / g.kb;
+                    const int64_t bo = colB[g.bcol0 + (int64_t)(n0 + nn) * nkb + blk];
+                    off = bo + (k - blk * g.kb);
+                }
+                if (off >= 0) {
+                    const V b = B[off];
+                    re = (R)b.x;
+                    im = (R)b.y;
+                }
             }
             Br[kk][nn] = re;
             Bi[kk][nn] = im;
@@ -534,13 +552,13 @@ hipError_t launch_stage_t(const DevPlan& d, const Stage& s, int nrhs, hipStream_
         if (d.mixed) {
             if (s.prec == PREC_SOLVE)
                 gemm_kernel<double2, float2, double><<<dim3(n, nrhs), 256, 0, st>>>(
-                    d.groups, d.tiles, d.colB, d.colC, d.opmat, d.bt, s.begin);
+                    d.groups, d.tiles, d.colB, d.colC, d.colA, d.opmat, d.bt, s.begin);
             else
                 gemm_kernel<float2, float2, float><<<dim3(n, nrhs), 256, 0, st>>>(
-                    d.groups, d.tiles, d.colB, d.colC, d.opmat32, d.bt, s.begin);
+                    d.groups, d.tiles, d.colB, d.colC, d.colA, d.opmat32, d.bt, s.begin);
         } else {
             gemm_kernel<double2, double2, double><<<dim3(n, nrhs), 256, 0, st>>>(
-                d.groups, d.tiles, d.colB, d.colC, d.opmat, d.bt, s.begin);
+                d.groups, d.tiles, d.colB, d.colC, d.colA, d.opmat, d.bt, s.begin);
         }
         break;
     case ST_FFTFWD:

@@ -47,6 +47,10 @@ struct KTask {
 };
 
 // grouped complex GEMM: C[:, ccol] (=, +=, atomic +=) diag(scale) * op(A) * B[:, bcol]
+// B columns: contiguous (kb = 0, offset colB[bcol0 + n]) or blocked along K in blocks of kb
+// rows, each block gathered from its own offset colB[bcol0 + n * K/kb + block], with the
+// matching A column block at colA[acol0 + block] (used to concatenate the non-empty
+// children of a box along K without padding).
 struct GGroup {
     int64_t aoff;   // A offset in the operator table
     int64_t soff;   // row scale (length M) offset, or -1
@@ -54,7 +58,9 @@ struct GGroup {
     int bbuf, cbuf;
     int ncols, col0;
     int mode;       // 0 store, 1 add, 2 atomic add
-    int pad;
+    int kb;         // K block size of the blocked gather (0 = contiguous)
+    int bcol0;      // first B offset in colB
+    int acol0;      // kb > 0: first A block offset in colA (block b at colA[acol0 + b])
 };
 struct GTile {
     int g, mt, nt, pad;

@@ -26,6 +26,22 @@ namespace {
 
 const double EPS = 1e-12;
 
+// C = A * B for column-major complex matrices (setup-time products of solve factors)
+CMat matmul(const CMat& A, const CMat& B) {
+    if (A.n != B.m) throw Error("internal: factor size mismatch");
+    CMat C;
+    C.m = A.m; C.n = B.n;
+    C.a.assign((size_t)C.m * C.n, cpx(0, 0));
+    for (int j = 0; j < B.n; ++j)
+        for (int k = 0; k < A.n; ++k) {
+            const cpx b = B.a[k + (size_t)j * B.m];
+            const cpx* a = &A.a[(size_t)k * A.m];
+            cpx* c = &C.a[(size_t)j * C.m];
+            for (int i = 0; i < A.m; ++i) c[i] += a[i] * b;
+        }
+    return C;
+}
+
 struct Builder {
     const Params& prm;
     const Mlib& mlib;
@@ -62,13 +78,19 @@ struct Builder {
         }
         return off;
     }
+    // B offsets: ncols entries (kb = 0, contiguous columns) or ncols x ceil(K/kb) block
+    // offsets (kb > 0, -1 = zero block); C offsets: ncols entries
     int add_group(GGroup g, const std::vector<int64_t>& bc, const std::vector<int64_t>& cc,
-                  double* flops) {
-        if (bc.empty()) return -1;
-        g.ncols = (int)bc.size();
-        g.col0 = (int)pl.colB.size();
-        pl.colB.insert(pl.colB.end(), bc.begin(), bc.end());
+                  double* flops, int kb = 0, const std::vector<int64_t>& ablk = {}) {
+        if (cc.empty()) return -1;
+        g.kb = kb;
+        g.acol0 = (int)pl.colA.size();
+        pl.colA.insert(pl.colA.end(), ablk.begin(), ablk.end());
+        g.ncols = (int)cc.size();
+        g.col0 = (int)pl.colC.size();
         pl.colC.insert(pl.colC.end(), cc.begin(), cc.end());
+        g.bcol0 = (int)pl.colB.size();
+        pl.colB.insert(pl.colB.end(), bc.begin(), bc.end());
         int gi = (int)pl.groups.size();
         pl.groups.push_back(g);
         int mt = (g.M + GEMM_TM - 1) / GEMM_TM, nt = (g.ncols + GEMM_TN - 1) / GEMM_TN;
@@ -88,8 +110,8 @@ struct Builder {
         // solves: LF pinv factors and HF E1/E2/E3 chains
         const std::string nm(name);
         s.prec = (nm.find("uc2ue") != std::string::npos || nm.find("dc2de") != std::string::npos ||
-                  nm.find("_e1") != std::string::npos || nm.find("_e2") != std::string::npos ||
-                  nm.find("_e3") != std::string::npos) ? PREC_SOLVE : PREC_TRANSLATION;
+                  nm.find("hf_up_e") != std::string::npos || nm.find("hf_dn_e") != std::string::npos)
+                     ? PREC_SOLVE : PREC_TRANSLATION;
         pl.stages.push_back(s);
     }
     int64_t gen(int64_t toff, int m, const double ts[3], int tneg, int64_t soff, int n,
@@ -120,7 +142,7 @@ struct HfDir {          // one (width, direction) group
     int64_t uep_g, ucp_g, uepc_g;   // generator point offsets (shuffled)
     int64_t ue2uc[8];   // generated matrices (relative offsets)
     int64_t de2dc[8];   // W = 1 only: G(uep_LF(W/2) + s_c, -ucp) (not a transpose of ue2uc)
-    int64_t e1, e2, e3;
+    int64_t e1;         // E1 * E2 * E3, r1 x r2
     std::vector<BoxKey> out, in;
 };
 
@@ -177,7 +199,7 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
         const LowFreqEntry* le;
         int r;
         int64_t uep_p, ucp_p;          // kapply point offsets
-        int64_t V, invs, UH, tab;      // opmat offsets
+        int64_t pinv, tab;             // opmat offsets
         int64_t ue2uc[8];              // generated
         int grid;
         bool has_children;
@@ -194,9 +216,13 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
         w.r = le.uc2ue[1].m;
         w.uep_p = add_pts(le.uep);
         w.ucp_p = add_pts(le.ucp);
-        w.V = upload(le.uc2ue[0]);
-        w.invs = upload(le.uc2ue[1]);
-        w.UH = upload(le.uc2ue[2]);
+        {
+            // check -> equivalent solve V * diag(1/s) * U^H as one FP64 matrix (NL x NL)
+            CMat Vs = le.uc2ue[0];
+            for (int j = 0; j < Vs.n; ++j)
+                for (int i = 0; i < Vs.m; ++i) Vs.a[i + (size_t)j * Vs.m] *= le.uc2ue[1].a[j];
+            w.pinv = upload(matmul(Vs, le.uc2ue[2]));
+        }
         w.tab = (int64_t)pl.opmat.size();
         pl.opmat.resize(pl.opmat.size() + (size_t)343 * G3, cpx(0, 0));
         for (auto& t : le.ue2dc) {
@@ -273,14 +299,11 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
             auto key = std::make_pair(W, h.srt);
             auto it = hf_entry_off.find(key);
             if (it == hf_entry_off.end()) {
-                int64_t e1 = upload(h.he->uc2ue[0]);
-                int64_t e2 = upload(h.he->uc2ue[1]);
-                upload(h.he->uc2ue[2]);
-                it = hf_entry_off.emplace(key, std::make_pair(e1, e2)).first;
+                // E1 * E2 * E3 (r1 x r2) as one FP64 matrix
+                int64_t e = upload(matmul(matmul(h.he->uc2ue[0], h.he->uc2ue[1]), h.he->uc2ue[2]));
+                it = hf_entry_off.emplace(key, std::make_pair(e, e)).first;
             }
             h.e1 = it->second.first;
-            h.e2 = it->second.second;
-            h.e3 = h.e2 + (int64_t)h.r1 * h.r2;
             // shuffled point sets and child kernels (UpwardHighFetch, mlib3d.cpp:176-224)
             std::vector<double> uep = Mlib::shuffled(h.prm, h.sgn, h.he->uep);
             std::vector<double> ucp = Mlib::shuffled(h.prm, h.sgn, h.he->ucp);
@@ -297,10 +320,14 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
             h.r1c = (int)uepc.size() / 3;
             h.uepc_g = genpts(uepc);
             double zero[3] = {0, 0, 0};
-            for (int c = 0; c < 8; ++c) {
+            for (int c = 0; c < 8; ++c) {       // contiguous: one r2 x 8 r1c matrix
                 double s[3];
                 shift_of(c, W, s);
                 h.ue2uc[c] = gen(h.ucp_g, h.r2, zero, 0, h.uepc_g, h.r1c, s);
+            }
+            for (int c = 0; c < 8; ++c) {
+                double s[3];
+                shift_of(c, W, s);
                 h.de2dc[c] = -1;
                 if (W == 1.0) {
                     // DownwardHighFetch at W = 1 (mlib3d.cpp:262-265): child check points are
@@ -366,16 +393,15 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
     pl.bufsize[BUF_VAL] = N;
     pl.bufsize[BUF_LF_UPEQ] = nlf * NL;
     pl.bufsize[BUF_LF_CHK] = nlf * NL;
-    pl.bufsize[BUF_LF_TMP] = nlf * NL;
+    pl.bufsize[BUF_LF_TMP] = 0;                 // unused since the solves are single matrices
     pl.bufsize[BUF_LF_DNEQ] = nlf * NL;
     pl.bufsize[BUF_HF_UP] = hf_up_size;
     pl.bufsize[BUF_HF_DN] = hf_dn_size;
     pl.bufsize[BUF_HF_A] = scr_size;
-    pl.bufsize[BUF_HF_B] = scr_size;
+    pl.bufsize[BUF_HF_B] = 0;
 
     // ================= stages
     stage(ST_GATHER, 0, 0, "gather");
-    stage(ST_ZERO, 0, 0, "zero_lfchk", BUF_LF_CHK, pl.bufsize[BUF_LF_CHK]);
     stage(ST_ZERO, 0, 0, "zero_hfdn", BUF_HF_DN, pl.bufsize[BUF_HF_DN]);
 
     // ---- LF upward (W ascending = finest first), EvalUpwardLow 418-490
@@ -401,20 +427,31 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
         stage(ST_KAPPLY, t0, (int)pl.ktasks.size(), "lf_s2m");
         int g0 = (int)pl.tiles.size();
         if (w.has_children) {
-            for (int c = 0; c < 8; ++c) {
-                std::vector<int64_t> bc, cc;
-                for (auto& k : kv.second) {
-                    if (boxes.at(k).terminal()) continue;
-                    BoxKey ck = Tree::child_key(k, c);
-                    auto it = lfidx.find(ck);
+            // one GEMM per child-occupancy mask: the present children's kernels side by side
+            // (NL x |mask| NL) times their equivalent densities stacked along K
+            std::map<int, std::pair<std::vector<int64_t>, std::vector<int64_t>>> bymask;
+            for (auto& k : kv.second) {
+                if (boxes.at(k).terminal()) continue;
+                int mask = 0;
+                std::vector<int64_t> ch;
+                for (int c = 0; c < 8; ++c) {
+                    auto it = lfidx.find(Tree::child_key(k, c));
                     if (it == lfidx.end()) continue;
-                    bc.push_back(it->second * NL);
-                    cc.push_back(lslot(k));
+                    mask |= 1 << c;
+                    ch.push_back(it->second * NL);
                 }
+                auto& e = bymask[mask];
+                e.first.insert(e.first.end(), ch.begin(), ch.end());
+                e.second.push_back(lslot(k));
+            }
+            for (auto& mk : bymask) {
+                std::vector<int64_t> ab;
+                for (int c = 0; c < 8; ++c)
+                    if (mk.first & (1 << c)) ab.push_back(G_(w.ue2uc[c]));
                 GGroup g{};
-                g.aoff = G_(w.ue2uc[c]); g.soff = -1; g.M = NL; g.K = NL; g.lda = NL; g.transA = 0;
-                g.bbuf = BUF_LF_UPEQ; g.cbuf = BUF_LF_CHK; g.mode = 2;
-                add_group(g, bc, cc, nullptr);
+                g.aoff = -1; g.soff = -1; g.M = NL; g.K = (int)ab.size() * NL; g.lda = NL; g.transA = 0;
+                g.bbuf = BUF_LF_UPEQ; g.cbuf = BUF_LF_CHK; g.mode = 0;
+                add_group(g, mk.second.first, mk.second.second, nullptr, NL, ab);
             }
         }
         stage(ST_GEMM, g0, (int)pl.tiles.size(), "lf_m2m");
@@ -422,60 +459,60 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
         for (auto& k : kv.second) all.push_back(lslot(k));
         g0 = (int)pl.tiles.size();
         GGroup g{};
-        g.aoff = w.UH; g.soff = w.invs; g.M = w.r; g.K = NL; g.lda = w.r; g.transA = 0;
-        g.bbuf = BUF_LF_CHK; g.cbuf = BUF_LF_TMP; g.mode = 0;
+        g.aoff = w.pinv; g.soff = -1; g.M = NL; g.K = NL; g.lda = NL; g.transA = 0;
+        g.bbuf = BUF_LF_CHK; g.cbuf = BUF_LF_UPEQ; g.mode = 0;
         add_group(g, all, all, nullptr);
-        stage(ST_GEMM, g0, (int)pl.tiles.size(), "lf_uc2ue_1");
-        g0 = (int)pl.tiles.size();
-        g.aoff = w.V; g.soff = -1; g.M = NL; g.K = w.r; g.lda = NL;
-        g.bbuf = BUF_LF_TMP; g.cbuf = BUF_LF_UPEQ;
-        add_group(g, all, all, nullptr);
-        stage(ST_GEMM, g0, (int)pl.tiles.size(), "lf_uc2ue_2");
+        stage(ST_GEMM, g0, (int)pl.tiles.size(), "lf_uc2ue");
     }
     stage(ST_ZERO, 0, 0, "zero_lfchk", BUF_LF_CHK, pl.bufsize[BUF_LF_CHK]);
 
     // ---- HF upward (W = 1, 2, ...), EvalUpwardHigh 773-854
     for (auto& lv : hfl) {
         double W = lv.first;
-        stage(ST_ZERO, 0, 0, "zero_hfa", BUF_HF_A, pl.bufsize[BUF_HF_A]);
         int g0 = (int)pl.tiles.size();
         for (auto& h : lv.second) {
             Index3 pd = W == 1.0 ? Index3() : parent_dir(h.dir);
-            for (int c = 0; c < 8; ++c) {
-                std::vector<int64_t> bc, cc;
-                for (auto& k : h.out) {
+            // one GEMM per (direction, child-occupancy mask): present children's kernels
+            // side by side (r2 x |mask| r1c), their equivalent densities stacked along K
+            std::map<int, std::pair<std::vector<int64_t>, std::vector<int64_t>>> bymask;
+            for (auto& k : h.out) {
+                int mask = 0;
+                std::vector<int64_t> ch;
+                for (int c = 0; c < 8; ++c) {
                     BoxKey ck = Tree::child_key(k, c);
                     auto bi = boxes.find(ck);
                     if (bi == boxes.end() || !bi->second.has_pts()) continue;
-                    if (W == 1.0) bc.push_back(lslot(ck));
-                    else bc.push_back(outslot.at({ck, pd}));
-                    cc.push_back(oscr.at({k, h.dir}));
+                    mask |= 1 << c;
+                    ch.push_back(W == 1.0 ? lslot(ck) : outslot.at({ck, pd}));
                 }
+                auto& e = bymask[mask];
+                e.first.insert(e.first.end(), ch.begin(), ch.end());
+                e.second.push_back(oscr.at({k, h.dir}));
+            }
+            for (auto& mk : bymask) {
+                std::vector<int64_t> ab;
+                for (int c = 0; c < 8; ++c)
+                    if (mk.first & (1 << c)) ab.push_back(G_(h.ue2uc[c]));
                 GGroup g{};
-                g.aoff = G_(h.ue2uc[c]); g.soff = -1; g.M = h.r2; g.K = h.r1c; g.lda = h.r2; g.transA = 0;
-                g.bbuf = W == 1.0 ? BUF_LF_UPEQ : BUF_HF_UP; g.cbuf = BUF_HF_A; g.mode = 2;
-                add_group(g, bc, cc, nullptr);
+                g.aoff = -1; g.soff = -1; g.M = h.r2; g.K = (int)ab.size() * h.r1c; g.lda = h.r2; g.transA = 0;
+                g.bbuf = W == 1.0 ? BUF_LF_UPEQ : BUF_HF_UP; g.cbuf = BUF_HF_A; g.mode = 0;
+                add_group(g, mk.second.first, mk.second.second, nullptr, h.r1c, ab);
             }
         }
         stage(ST_GEMM, g0, (int)pl.tiles.size(), "hf_u2u");
-        const char* nm[3] = {"hf_up_e3", "hf_up_e2", "hf_up_e1"};
-        for (int step = 0; step < 3; ++step) {
-            g0 = (int)pl.tiles.size();
-            for (auto& h : lv.second) {
-                std::vector<int64_t> bc, cc;
-                for (auto& k : h.out) {
-                    bc.push_back(oscr.at({k, h.dir}));
-                    cc.push_back(step == 2 ? outslot.at({k, h.dir}) : oscr.at({k, h.dir}));
-                }
-                GGroup g{};
-                g.soff = -1; g.transA = 0; g.mode = 0;
-                if (step == 0) { g.aoff = h.e3; g.M = h.r2; g.K = h.r2; g.lda = h.r2; g.bbuf = BUF_HF_A; g.cbuf = BUF_HF_B; }
-                if (step == 1) { g.aoff = h.e2; g.M = h.r1; g.K = h.r2; g.lda = h.r1; g.bbuf = BUF_HF_B; g.cbuf = BUF_HF_A; }
-                if (step == 2) { g.aoff = h.e1; g.M = h.r1; g.K = h.r1; g.lda = h.r1; g.bbuf = BUF_HF_A; g.cbuf = BUF_HF_UP; }
-                add_group(g, bc, cc, nullptr);
+        g0 = (int)pl.tiles.size();
+        for (auto& h : lv.second) {
+            std::vector<int64_t> bc, cc;
+            for (auto& k : h.out) {
+                bc.push_back(oscr.at({k, h.dir}));
+                cc.push_back(outslot.at({k, h.dir}));
             }
-            stage(ST_GEMM, g0, (int)pl.tiles.size(), nm[step]);
+            GGroup g{};
+            g.aoff = h.e1; g.soff = -1; g.M = h.r1; g.K = h.r2; g.lda = h.r1; g.transA = 0; g.mode = 0;
+            g.bbuf = BUF_HF_A; g.cbuf = BUF_HF_UP;
+            add_group(g, bc, cc, nullptr);
         }
+        stage(ST_GEMM, g0, (int)pl.tiles.size(), "hf_up_e");
     }
 
     // ---- HF downward (W = max .. 1), EvalDownwardHigh 996-1040
@@ -492,24 +529,20 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
             pl.stats.m2l_groups++;
         }
         stage(ST_GEMM, g0, (int)pl.tiles.size(), "hf_m2l");
-        const char* nm[3] = {"hf_dn_e1t", "hf_dn_e2t", "hf_dn_e3t"};
-        for (int step = 0; step < 3; ++step) {
-            g0 = (int)pl.tiles.size();
-            for (auto& h : lt->second) {
-                std::vector<int64_t> bc, cc;
-                for (auto& k : h.in) {
-                    bc.push_back(step == 0 ? inslot.at({k, h.dir}) : iscr.at({k, h.dir}));
-                    cc.push_back(iscr.at({k, h.dir}));
-                }
-                GGroup g{};
-                g.soff = -1; g.transA = 1; g.mode = 0;
-                if (step == 0) { g.aoff = h.e1; g.M = h.r1; g.K = h.r1; g.lda = h.r1; g.bbuf = BUF_HF_DN; g.cbuf = BUF_HF_A; }
-                if (step == 1) { g.aoff = h.e2; g.M = h.r2; g.K = h.r1; g.lda = h.r1; g.bbuf = BUF_HF_A; g.cbuf = BUF_HF_B; }
-                if (step == 2) { g.aoff = h.e3; g.M = h.r2; g.K = h.r2; g.lda = h.r2; g.bbuf = BUF_HF_B; g.cbuf = BUF_HF_A; }
-                add_group(g, bc, cc, nullptr);
+        g0 = (int)pl.tiles.size();
+        for (auto& h : lt->second) {
+            std::vector<int64_t> bc, cc;
+            for (auto& k : h.in) {
+                bc.push_back(inslot.at({k, h.dir}));
+                cc.push_back(iscr.at({k, h.dir}));
             }
-            stage(ST_GEMM, g0, (int)pl.tiles.size(), nm[step]);
+            // (E1 E2 E3)^T = E3^T E2^T E1^T (DownwardHighFetch transposes, mlib3d.cpp:247-252)
+            GGroup g{};
+            g.aoff = h.e1; g.soff = -1; g.M = h.r2; g.K = h.r1; g.lda = h.r1; g.transA = 1; g.mode = 0;
+            g.bbuf = BUF_HF_DN; g.cbuf = BUF_HF_A;
+            add_group(g, bc, cc, nullptr);
         }
+        stage(ST_GEMM, g0, (int)pl.tiles.size(), "hf_dn_et");
         g0 = (int)pl.tiles.size();
         for (auto& h : lt->second) {
             Index3 pd = W == 1.0 ? Index3() : parent_dir(h.dir);
@@ -615,15 +648,10 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
         for (auto& k : lt->second) all.push_back(lslot(k));
         int g0 = (int)pl.tiles.size();
         GGroup g{};
-        g.aoff = w.V; g.soff = w.invs; g.M = w.r; g.K = NL; g.lda = NL; g.transA = 1;
-        g.bbuf = BUF_LF_CHK; g.cbuf = BUF_LF_TMP; g.mode = 0;
+        g.aoff = w.pinv; g.soff = -1; g.M = NL; g.K = NL; g.lda = NL; g.transA = 1;
+        g.bbuf = BUF_LF_CHK; g.cbuf = BUF_LF_DNEQ; g.mode = 0;
         add_group(g, all, all, nullptr);
-        stage(ST_GEMM, g0, (int)pl.tiles.size(), "lf_dc2de_1");
-        g0 = (int)pl.tiles.size();
-        g.aoff = w.UH; g.soff = -1; g.M = NL; g.K = w.r; g.lda = w.r; g.transA = 1;
-        g.bbuf = BUF_LF_TMP; g.cbuf = BUF_LF_DNEQ;
-        add_group(g, all, all, nullptr);
-        stage(ST_GEMM, g0, (int)pl.tiles.size(), "lf_dc2de_2");
+        stage(ST_GEMM, g0, (int)pl.tiles.size(), "lf_dc2de");
         if (w.has_children) {
             g0 = (int)pl.tiles.size();
             for (int c = 0; c < 8; ++c) {

@@ -38,7 +38,7 @@ struct HostPlan {
     std::vector<KSrc> ksrcs;
     std::vector<GGroup> groups;
     std::vector<GTile> tiles;
-    std::vector<int64_t> colB, colC;
+    std::vector<int64_t> colB, colC, colA;
     std::vector<FftSrc> fsrc;
     std::vector<VTarget> vtrg;
     std::vector<VPair> vpair;

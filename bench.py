@@ -60,6 +60,44 @@ def cpu_baseline(ops_paths, kernel):
                       f"one RHS, {dt:.1f} s"}
 
 
+def serve_bench(names, qps, count, slot, precision, seed=4):
+    """Poisson request stream over resident geometries (continuous batching)."""
+    from ddfmm_amd import geometry, mlib_gen
+    from ddfmm_amd.serve import GpuEngine, Server, latency_stats, poisson_requests
+    from ddfmm_amd.wave3d import FP64, MIXED
+
+    ops = mlib_gen.ensure_library(os.path.join(ROOT, "ddfmm_amd", "_ops"), ktype=0, accu=2, NPQ=4,
+                                  hf_max_W=4, lf_min_W=2.0 ** -3)
+    pts = {n: geometry.workload_points(n) for n in names}
+
+    def make(key):
+        wl = geometry.WORKLOADS[key]
+        return GpuEngine(key, pts[key], {"-wave3d_ACCU": 2, "-wave3d_K": wl["K"], "-wave3d_ptsmax": wl["ptsmax"],
+                                         "-wave3d_maxlevel": 14}, ops, slot,
+                         MIXED if precision == "mixed" else FP64, numC=wl.get("numC", 1))
+
+    srv = Server(make, lambda key: 0, hbm_budget=200 << 30)
+    t0 = time.perf_counter()
+    for n in names:                     # resident before the stream starts (warm cache)
+        srv.engines[n] = make(n)
+    setup_s = time.perf_counter() - t0
+    warm = poisson_requests({n: len(pts[n]) for n in names}, qps=qps, count=len(names), seed=seed + 1)
+    srv.run(warm)
+    reqs = poisson_requests({n: len(pts[n]) for n in names}, qps=qps, count=count, seed=seed)
+    t0 = time.perf_counter()
+    srv.run(reqs)
+    wall = time.perf_counter() - t0
+    st = latency_stats(reqs)
+    evals = sum(r.nrhs * len(pts[r.geometry]) for r in reqs if r.error is None)
+    out = {"geometries": names, "qps": qps, "requests": count, "rhs_per_request": "1-16", "slot_rhs": slot,
+           "batches": srv.batches, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st.get("mean_ms"),
+           "throughput_M_point_evals_per_s": evals / wall / 1e6, "wall_s": wall, "setup_s": setup_s,
+           "errors": sum(r.error is not None for r in reqs),
+           "resident_GiB": srv.resident_bytes() / 2**30}
+    srv.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +111,10 @@ def main():
                     help="one 256 MiB device-to-device copy (blit kernel) to calibrate PMC byte counters")
     ap.add_argument("--chk", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve", default="sphere8_60k,sphere16_250k_x64",
+                    help="comma-separated geometries for the serving leg ('' to skip)")
+    ap.add_argument("--serve-qps", type=float, default=40.0)
+    ap.add_argument("--serve-requests", type=int, default=80)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -189,6 +231,17 @@ def main():
     hip.free(vd)
     w.close()
 
+    serve = None
+    if args.serve:
+        log("serving leg ...")
+        try:
+            serve = serve_bench(args.serve.split(","), args.serve_qps, args.serve_requests, 16, args.precision)
+            log(f"serving: p50 {serve['p50_ms']:.2f} ms p99 {serve['p99_ms']:.2f} ms, "
+                f"{serve['throughput_M_point_evals_per_s']:.1f} M point-evals/s, {serve['batches']} batches")
+        except Exception as e:  # the serving leg must not hide the main metric
+            serve = {"error": repr(e)}
+            log(f"serving leg failed: {e!r}")
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
@@ -216,6 +269,7 @@ def main():
                    "meets_tolerance": bool(err <= 1e-5)},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "serve": serve,
         "per_kernel_ms": {k: round(v["ms"], 4) for k, v in per.items()},
     }
     print(json.dumps(out), flush=True)

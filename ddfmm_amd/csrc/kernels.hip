@@ -86,24 +86,40 @@ __device__ __forceinline__ float2 green<float2>(double dx, double dy, double dz,
 template <class V>
 __device__ __forceinline__ V* bufp(const BufTable& bt, int b) { return (V*)bt.p[b]; }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2): give
+// XCD x the contiguous chunk [x*q + min(x, r), ...) of the n work items (q = n/8, r = n%8)
+// so that neighbouring items (Morton-ordered boxes, tiles of one operator) share an L2.
+__device__ __forceinline__ int xcd_item(int b, int n) {
+    const int x = b & 7, k = b >> 3;
+    const int q = n >> 3, r = n & 7;
+    return x * q + min(x, r) + k;
+}
+
 // ---------------------------------------------------------------- kernel apply
 constexpr int KA_THREADS = 128;
 
+// One block per task.  A pass covers nt = min(count, 128) targets; when nt < 128 the
+// 128 threads split into G = 128 / nt source groups (thread = target x group) that
+// stride through each LDS source tile, and the partial sums are reduced through LDS.
 template <class V, int RC>
 __global__ __launch_bounds__(KA_THREADS) void kapply_kernel(
     const KTask* __restrict__ tasks, const KSrc* __restrict__ srcs,
     const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
     BufTable bt, int task0, int nrhs, int ktype) {
-    const KTask t = tasks[task0 + blockIdx.x];
+    const KTask t = tasks[task0 + xcd_item(blockIdx.x, gridDim.x)];
     const int r0 = blockIdx.y * RC;
     const int nr = min(RC, nrhs - r0);
     __shared__ double sx[KA_THREADS], sy[KA_THREADS], sz[KA_THREADS];
     __shared__ V sv[RC][KA_THREADS];
+    __shared__ V red[RC][KA_THREADS];
     for (int tb = 0; tb < t.count; tb += KA_THREADS) {
-        const int i = tb + threadIdx.x;
-        const bool ok = i < t.count;
+        const int nt = min(KA_THREADS, t.count - tb);
+        const int G = KA_THREADS / nt;
+        const int ti = threadIdx.x % nt, gi = threadIdx.x / nt;
+        const bool act = gi < G;
+        const int i = tb + ti;
         double tx = 0, ty = 0, tz = 0;
-        if (ok) {
+        if (act) {
             tx = px[t.off + i] * t.scale + t.c[0];
             ty = py[t.off + i] * t.scale + t.c[1];
             tz = pz[t.off + i] * t.scale + t.c[2];
@@ -128,8 +144,8 @@ __global__ __launch_bounds__(KA_THREADS) void kapply_kernel(
                         sv[q][threadIdx.x] = q < nr ? vb[(int64_t)(r0 + q) * vs + sb + threadIdx.x] : mk<V>(0, 0);
                 }
                 __syncthreads();
-                if (ok) {
-                    for (int j = 0; j < n; ++j) {
+                if (act) {
+                    for (int j = gi; j < n; j += G) {
                         const V g = green<V>(tx - sx[j], ty - sy[j], tz - sz[j], ktype);
 #pragma unroll
                         for (int q = 0; q < RC; ++q) cfma(acc[q], g, sv[q][j]);
@@ -137,7 +153,22 @@ __global__ __launch_bounds__(KA_THREADS) void kapply_kernel(
                 }
             }
         }
-        if (ok) {
+        if (G > 1) {                      // reduce the source groups of each target
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < RC; ++q) red[q][threadIdx.x] = acc[q];
+            __syncthreads();
+            if (gi == 0) {
+                for (int h = 1; h < G; ++h)
+#pragma unroll
+                    for (int q = 0; q < RC; ++q) {
+                        const V o = red[q][ti + h * nt];
+                        acc[q].x += o.x;
+                        acc[q].y += o.y;
+                    }
+            }
+        }
+        if (gi == 0) {
             V* ob = bufp<V>(bt, t.obuf) + t.ooff + i;
             const int64_t os = bt.stride[t.obuf];
 #pragma unroll
@@ -154,6 +185,7 @@ __global__ __launch_bounds__(KA_THREADS) void kapply_kernel(
                 }
             }
         }
+        __syncthreads();
     }
 }
 
@@ -195,7 +227,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(
     const int64_t* __restrict__ colB, const int64_t* __restrict__ colC, const int64_t* __restrict__ colA,
     const TA* __restrict__ opm, BufTable bt, int tile0) {
     using M = Mfma<R>;
-    const GTile tl = tiles[tile0 + blockIdx.x];
+    const GTile tl = tiles[tile0 + xcd_item(blockIdx.x, gridDim.x)];
     const GGroup g = groups[tl.g];
     const int rhs = blockIdx.y;
     const int m0 = tl.mt * GEMM_TM, n0 = tl.nt * GEMM_TN;
@@ -364,7 +396,7 @@ __global__ __launch_bounds__(256) void fft_fwd_kernel(const FftSrc* __restrict__
     __shared__ V g0[G3], g1[G3];
     __shared__ V tw[G];
     Dft<G, V>::twiddles(tw);
-    const FftSrc f = fs[f0 + blockIdx.x];
+    const FftSrc f = fs[f0 + xcd_item(blockIdx.x, gridDim.x)];
     const int rhs = blockIdx.y;
     const V* in = bufp<V>(bt, BUF_LF_UPEQ) + (int64_t)rhs * bt.stride[BUF_LF_UPEQ] + f.in_off;
     V* out = bufp<V>(bt, BUF_SPEC) + (int64_t)rhs * bt.stride[BUF_SPEC] + f.spec_off;
@@ -390,7 +422,7 @@ __global__ __launch_bounds__(256) void vlist_kernel(const VTarget* __restrict__ 
     __shared__ V g0[G3], g1[G3];
     __shared__ V tw[G];
     Dft<G, V>::twiddles(tw);
-    const VTarget t = vt[v0 + blockIdx.x];
+    const VTarget t = vt[v0 + xcd_item(blockIdx.x, gridDim.x)];
     const int rhs = blockIdx.y;
     const V* spec = bufp<V>(bt, BUF_SPEC) + (int64_t)rhs * bt.stride[BUF_SPEC];
     V acc[PER];

@@ -26,6 +26,21 @@ namespace {
 
 const double EPS = 1e-12;
 
+uint64_t morton(const BoxKey& k) {
+    uint64_t code = 0;
+    for (int b = 0; b < 20; ++b)
+        for (int d = 0; d < 3; ++d)
+            code |= (uint64_t)((k.second[d] >> b) & 1) << (3 * b + (2 - d));
+    return code;
+}
+
+std::vector<BoxKey> morton_sorted(std::vector<BoxKey> v) {
+    std::stable_sort(v.begin(), v.end(), [](const BoxKey& a, const BoxKey& b) {
+        return a.first != b.first ? a.first < b.first : morton(a) < morton(b);
+    });
+    return v;
+}
+
 // C = A * B for column-major complex matrices (setup-time products of solve factors)
 CMat matmul(const CMat& A, const CMat& B) {
     if (A.n != B.m) throw Error("internal: factor size mismatch");
@@ -596,7 +611,7 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
         spec_max = std::max(spec_max, (int64_t)specidx.size());
         stage(ST_FFTFWD, f0, (int)pl.fsrc.size(), "lf_fft");
         int v0 = (int)pl.vtrg.size();
-        for (auto& k : lt->second) {
+        for (auto& k : morton_sorted(lt->second)) {
             const Box& b = boxes.at(k);
             if (b.V.empty()) continue;
             double ct[3];
@@ -677,7 +692,7 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
 
     // ---- leaf kernel: U list, W list, X list (direct), L2T (eval 550-560, 585-718)
     int t0 = (int)pl.ktasks.size();
-    for (auto& k : leaves) {
+    for (auto& k : morton_sorted(leaves)) {
         const Box& b = boxes.at(k);
         double W = tree.width(k);
         LfW& w = lfw[W];

@@ -1,0 +1,56 @@
+"""Serving on the GPU: continuous batching over two resident geometries; every result is
+the same alone and in the mix (mixed precision: FP32 atomics reorder sums, tolerance
+1e-5 relative), and all results stay within 1e-5 of the FP64 direct sum's accuracy bar."""
+import numpy as np
+import pytest
+
+from ddfmm_amd import geometry
+from ddfmm_amd.serve import GpuEngine, Request, Server, latency_stats, poisson_requests
+from ddfmm_amd.wave3d import MIXED
+
+pytestmark = pytest.mark.gpu
+
+GEOMS = {"s4": (4.0, 5000, 11), "s8": (8.0, 20000, 12)}
+
+
+def _engines(oplib, slot=8):
+    pts = {k: geometry.sphere(n, K, seed) for k, (K, n, seed) in GEOMS.items()}
+
+    def make(key):
+        K = GEOMS[key][0]
+        return GpuEngine(key, pts[key], {"-wave3d_ACCU": 2, "-wave3d_K": K, "-wave3d_maxlevel": 12},
+                         oplib, slot, MIXED)
+
+    return pts, make
+
+
+def test_alone_equals_in_mix(oplib):
+    pts, make = _engines(oplib)
+    srv = Server(make, lambda key: 1 << 30, hbm_budget=64 << 30)
+    reqs = poisson_requests({k: v[1] for k, v in GEOMS.items()}, qps=400, count=24, seed=5, max_rhs=4)
+    srv.run(reqs)
+    st = latency_stats(reqs)
+    print(f"served {st['served']} batches {srv.batches} p50 {st['p50_ms']:.2f} ms p99 {st['p99_ms']:.2f} ms",
+          flush=True)
+    assert all(r.error is None for r in reqs)
+    assert srv.batches < len(reqs)                 # requests were batched together
+    # the same requests served one at a time on the same resident engines
+    for r in reqs[:10]:
+        alone = Request(1000 + r.rid, r.geometry, r.den, 0.0)
+        srv.run([alone])
+        e = np.linalg.norm(alone.result - r.result) / np.linalg.norm(alone.result)
+        assert e < 1e-5, (r.rid, e)
+    srv.close()
+
+
+def test_eviction_under_budget(oplib):
+    pts, make = _engines(oplib, slot=4)
+    probe = make("s8")
+    b = probe.bytes
+    probe.close()
+    srv = Server(make, lambda key: b, hbm_budget=int(1.5 * b))   # only one engine resident at a time
+    reqs = poisson_requests({k: v[1] for k, v in GEOMS.items()}, qps=200, count=8, seed=6, max_rhs=2)
+    srv.run(reqs)
+    assert all(r.error is None for r in reqs)
+    assert srv.evictions >= 1
+    srv.close()

@@ -183,49 +183,84 @@ def read_hd(path):
 
 # ---- point / density / check-key files (tt.cpp:66-98, 169-177; new_data.m:158-242) ----
 
-def write_parvec_points(path, pos):
-    """ParVec<int,Point3,PtPrtn> on one rank: map<int,Point3> then PtPrtn ownerinfo.
+_PT = np.dtype([("k", "<i4"), ("x", "<f8", (3,))])     # packed (key, Point3) record
+_CX = np.dtype([("k", "<i4"), ("v", "<f8", (2,))])     # packed (key, cpx) record
 
-    Mirrors the per-rank file read by tt.cpp:66-82 (deserialize of the ParVec's local
-    map followed by the partition vector ``[0, N]``)."""
-    pos = np.asarray(pos, dtype=np.float64)
-    buf = io.BytesIO()
-    w = _W(buf)
-    w.i32(len(pos))
-    for k, p in enumerate(pos):
-        w.i32(k)
-        for d in range(3):
-            w.f64(p[d])
-    w.i32(2)
-    w.i32(0)
-    w.i32(len(pos))
+
+def _write_parvec(path, rec, n):
     with open(path, "wb") as f:
-        f.write(buf.getvalue())
+        f.write(struct.pack("<i", n))
+        f.write(rec.tobytes())
+        f.write(struct.pack("<iii", 2, 0, n))          # PtPrtn ownerinfo [0, N] (one rank)
+
+
+def write_parvec_points(path, pos):
+    """ParVec<int,Point3,PtPrtn> on one rank: map<int,Point3> (size, then key/value pairs)
+    followed by the PtPrtn owner vector [0, N] (tt.cpp:66-82, parvec.hpp:530-550)."""
+    pos = np.asarray(pos, dtype=np.float64)
+    rec = np.empty(len(pos), dtype=_PT)
+    rec["k"] = np.arange(len(pos))
+    rec["x"] = pos
+    _write_parvec(path, rec, len(pos))
 
 
 def write_parvec_cpx(path, den):
     den = np.asarray(den, dtype=np.complex128)
-    buf = io.BytesIO()
-    w = _W(buf)
-    w.i32(len(den))
-    for k, v in enumerate(den):
-        w.i32(k)
-        w.f64(v.real)
-        w.f64(v.imag)
-    w.i32(2)
-    w.i32(0)
-    w.i32(len(den))
-    with open(path, "wb") as f:
-        f.write(buf.getvalue())
+    rec = np.empty(len(den), dtype=_CX)
+    rec["k"] = np.arange(len(den))
+    rec["v"] = den.view(np.float64).reshape(-1, 2)
+    _write_parvec(path, rec, len(den))
+
+
+def _read_parvec(path, dt):
+    with open(path, "rb") as f:
+        buf = f.read()
+    n = struct.unpack_from("<i", buf, 0)[0]
+    rec = np.frombuffer(buf, dtype=dt, count=n, offset=4)
+    p = 4 + n * dt.itemsize
+    m = struct.unpack_from("<i", buf, p)[0]
+    own = list(struct.unpack_from(f"<{m}i", buf, p + 4))
+    return rec, own
 
 
 def read_parvec_cpx(path):
+    rec, _ = _read_parvec(path, _CX)
+    vals = rec["v"].copy().view(np.complex128).reshape(-1)
+    return rec["k"].astype(np.int64), vals
+
+
+def read_parvec_points(path):
+    """Inverse of write_parvec_points: (keys, positions (n,3), ownerinfo)."""
+    rec, own = _read_parvec(path, _PT)
+    return rec["k"].astype(np.int64), rec["x"].copy(), own
+
+
+def write_int_vec(path, v):
+    """IntNumVec (serialize.hpp:360-382): m then m ints."""
+    v = np.asarray(v, dtype=np.int32)
+    with open(path, "wb") as f:
+        f.write(struct.pack("<i", len(v)))
+        f.write(v.tobytes())
+
+
+def read_int_vec(path):
     with open(path, "rb") as f:
         r = _R(f.read())
-    n = r.i32()
-    keys = np.empty(n, dtype=np.int64)
-    vals = np.empty(n, dtype=np.complex128)
-    for k in range(n):
-        keys[k] = r.i32()
-        vals[k] = complex(r.f64(), r.f64())
-    return keys, vals
+    m = r.i32()
+    return r._arr((m,), np.int32).astype(np.int64)
+
+
+def write_int_tns(path, t):
+    """IntNumTns (serialize.hpp:414-440): m, n, p then ints, first index fastest."""
+    t = np.asarray(t, dtype=np.int32)
+    with open(path, "wb") as f:
+        for s in t.shape:
+            f.write(struct.pack("<i", s))
+        f.write(np.asfortranarray(t).tobytes(order="F"))
+
+
+def read_int_tns(path):
+    with open(path, "rb") as f:
+        r = _R(f.read())
+    m, n, p = r.i32(), r.i32(), r.i32()
+    return r._arr((m, n, p), np.int32)

@@ -54,3 +54,28 @@ def test_eviction_under_budget(oplib):
     assert all(r.error is None for r in reqs)
     assert srv.evictions >= 1
     srv.close()
+
+
+def test_tt_driver_reference_formats(tmp_path):
+    """tt-style driver on files in the reference layout: RESULT block, Ea within 1e-5 and the
+    written val file equal to a direct evaluation through the API."""
+    import io
+
+    from ddfmm_amd import serialize, tt
+
+    root = tmp_path / "data"
+    pos, den, ld, hd = tt.make_data(str(root), "sphere8_60k", nchk=300)
+    buf = io.StringIO()
+    res = tt.run({"-dataroot": str(root), "-posfile": "pos", "-denfile": "den", "-geomprtn": "geomprtn",
+                  "-chkfile": "chk", "-valfile": "val", "-kernel": "0", "-mlib3d_ldname": ld,
+                  "-mlib3d_hdname": hd, "-wave3d_ACCU": "2", "-wave3d_NPQ": "4", "-wave3d_K": "8",
+                  "-wave3d_ptsmax": "100", "-wave3d_maxlevel": "12"}, out=buf)
+    text = buf.getvalue()
+    print(text, flush=True)
+    assert "RESULT" in text and res["Ea"] < 1e-5
+    keys, val = serialize.read_parvec_cpx(str(root / "val_0_1"))
+    assert np.array_equal(keys, np.arange(len(pos)))
+    from oracle import wave3d_ref as R
+    chk = geometry.check_targets(len(pos), 200, 0)
+    exact = R.direct_sum(pos[chk], pos, den)
+    assert np.linalg.norm(val[chk] - exact) / np.linalg.norm(exact) < 1e-5

@@ -106,7 +106,7 @@ def main():
     ap.add_argument("--workload", default="sphere32_1m")
     ap.add_argument("--kernel", type=int, default=0)
     ap.add_argument("--nrhs", type=int, default=None)
-    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed", "fp64_fused"])
     ap.add_argument("--calib-copy", action="store_true",
                     help="one 256 MiB device-to-device copy (blit kernel) to calibrate PMC byte counters")
     ap.add_argument("--chk", type=int, default=1000)
@@ -123,7 +123,7 @@ def main():
 
     from ddfmm_amd import geometry, mlib_gen
     from ddfmm_amd._lib import Hip
-    from ddfmm_amd.wave3d import FP64, MIXED, Kernel3d, Mlib3d, Wave3d, direct_sum
+    from ddfmm_amd.wave3d import FP64, FP64_FUSED, MIXED, Kernel3d, Mlib3d, Wave3d, direct_sum
 
     wl = dict(geometry.WORKLOADS[args.workload])
     nrhs = args.nrhs or wl["nrhs"]
@@ -142,7 +142,8 @@ def main():
     w.geomprtn = np.zeros((wl.get("numC", 1),) * 3, dtype=np.int32)
     t0 = time.perf_counter()
     w.setup({"-wave3d_ACCU": 2, "-wave3d_NPQ": 4, "-wave3d_K": wl["K"], "-wave3d_ptsmax": wl["ptsmax"],
-             "-wave3d_maxlevel": 14}, nrhs_max=nrhs, precision=MIXED if args.precision == "mixed" else FP64)
+             "-wave3d_maxlevel": 14}, nrhs_max=nrhs,
+            precision={"mixed": MIXED, "fp64": FP64, "fp64_fused": FP64_FUSED}[args.precision])
     st = w.stats()
     log(f"setup {time.perf_counter() - t0:.1f}s: leaves={st['nleaf']} m2l_pairs={st['m2l_pairs']} "
         f"v_pairs={st['v_pairs']} device={st['device_bytes'] / 2**30:.2f} GiB")
@@ -178,10 +179,14 @@ def main():
     # per-kernel time from HIP events on the evaluation stream
     w.set_timing(True)
     per = {}
+    per_stage = {}
     reps = 3
     for _ in range(reps):
         w.eval_device(dd, vd, nrhs)
         for name, ms, work, kind, prec in w.stage_times():
+            ps = per_stage.setdefault(name, [0.0, 0.0])
+            ps[0] += ms / reps
+            ps[1] += work * nrhs / reps
             k = kernel_class(kind, prec)
             e = per.setdefault(k, {"ms": 0.0, "launches": 0, "work": 0.0})
             e["ms"] += ms / reps
@@ -190,6 +195,9 @@ def main():
     for k in per:
         per[k]["launches"] //= reps
     w.set_timing(False)
+    log("top stages (ms, work/ms): " + ", ".join(
+        f"{n}={v[0]:.3f}({v[1] / max(v[0], 1e-9) / 1e9:.2f}G/ms)" for n, v in
+        sorted(per_stage.items(), key=lambda kv: -kv[1][0])[:12]))
     dom = max(per, key=lambda k: per[k]["ms"])
     t_dom = per[dom]["ms"] * 1e-3
     launches = per[dom]["launches"]
@@ -261,7 +269,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32+fp64 (FP32 translations/near field, FP64 solves)" if args.precision == "mixed" else "fp64",
+        "dtype": {"mixed": "fp32+fp64 (FP32 translations/near field, FP64 solves)", "fp64": "fp64",
+                  "fp64_fused": "fp64 (fused solves)"}[args.precision],
         "data": "synthetic",
         "config": {"workload": args.workload, "n_points": N, "nrhs": nrhs, "kernel": "exp(ikr)/r" if args.kernel == 0
                    else "exp(ikr)", "K": wl["K"], "ACCU": 2, "NPQ": 4, "ptsmax": wl["ptsmax"],

@@ -115,7 +115,7 @@ int ddfmm_setup(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_params* 
     if (p->ACCU < 1 || p->ACCU > 3) return fail(DDFMM_EINVAL, "ACCU must be 1, 2 or 3");
     if (p->ACCU != 2) return fail(DDFMM_EINVAL, "this build evaluates ACCU=2 (P=6) only");
     if (p->nrhs_max < 1) return fail(DDFMM_EINVAL, "nrhs_max must be >= 1");
-    if (p->precision != DDFMM_FP64 && p->precision != DDFMM_MIXED)
+    if (p->precision != DDFMM_FP64 && p->precision != DDFMM_MIXED && p->precision != DDFMM_FP64_FUSED)
         return fail(DDFMM_EINVAL, "unsupported precision mode");
     double lk = std::log2(p->K);
     if (!(p->K >= 1) || std::abs(lk - std::round(lk)) > 1e-12) return fail(DDFMM_EINVAL, "K must be a power of two >= 1");
@@ -131,6 +131,7 @@ int ddfmm_setup(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_params* 
         for (int d = 0; d < 3; ++d) q.ctr[d] = p->ctr[d];
         q.ACCU = p->ACCU; q.NPQ = p->NPQ; q.ptsmax = p->ptsmax; q.maxlevel = p->maxlevel;
         q.numC = p->numC; q.ktype = p->kernel;
+        q.fuse_solves = p->precision != DDFMM_FP64;
         c->nrhs_max = p->nrhs_max;
         c->precision = p->precision;
         Mlib mlib;
@@ -150,7 +151,7 @@ int ddfmm_setup(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_params* 
         DevPlan& d = c->dp;
         d = DevPlan();
         d.N = h.N; d.P = h.P; d.NL = h.NL; d.ktype = c->prm.ktype;
-        d.mixed = c->precision == DDFMM_MIXED ? 1 : 0;
+        d.mixed = c->precision == DDFMM_MIXED ? 1 : 0;     // FP64 and FP64_FUSED: FP64 arithmetic
         const int64_t esz = d.mixed ? (int64_t)sizeof(float2) : (int64_t)sizeof(double2);
         d.px = c->upload(h.px); d.py = c->upload(h.py); d.pz = c->upload(h.pz);
         d.perm = c->upload(h.perm);

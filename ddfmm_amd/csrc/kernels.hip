@@ -209,8 +209,7 @@ template <> struct Mfma<double> {
     static __device__ __forceinline__ int row(int lk, int v) { return lk + 4 * v; }
 };
 
-constexpr int GK = 16;           // k-slab staged in LDS
-constexpr int LDSP = GEMM_TM + 4;  // padded LDS row
+constexpr int MAXKB = 8;           // at most 8 K blocks (children of a box)
 
 template <class V>
 __device__ __forceinline__ void atomic_add(V* p, typename Real<V>::T x, typename Real<V>::T y) {
@@ -218,33 +217,122 @@ __device__ __forceinline__ void atomic_add(V* p, typename Real<V>::T x, typename
     unsafeAtomicAdd(&p->y, y);
 }
 
+// LDS layouts (bank-conflict-free operand reads for wave64 MFMA 16x16x4):
+//   A, not transposed: [k][m], row stride SA (f32: 80 = 16 mod 32 words; f64: 68 doubles)
+//   A transposed and B: [m|n][k], row stride SK (f32: KS + 2 = 34, f64: KS + 1 = 17)
+template <class R> struct Lds;
+template <> struct Lds<float> { static constexpr int KS = 32, SA = 80, SK = 34; };
+template <> struct Lds<double> { static constexpr int KS = 16, SA = 68, SK = 17; };
+
 // C[:, ccol] (=|+=|atomic +=) diag(scale) * op(A) * B[:, bcol]; block tile 64 x 64, four
-// waves of 32 x 32, each a 2 x 2 grid of 16 x 16 MFMA tiles; complex product as four
-// real MFMA products per k-step.
+// waves of 32 x 32, each a 2 x 2 grid of 16 x 16 MFMA tiles; complex product as four real
+// MFMA products per k-step.  K is staged in slabs of KS through a register-prefetch
+// pipeline: slab s+1 is loaded into registers while slab s runs on the matrix cores.
+// Blocked gathers (kb > 0, kb >= KS): the tile multiplies only the K blocks present in at
+// least one of its columns (compacted list in LDS).
 template <class TA, class V, class R>
 __global__ __launch_bounds__(256) void gemm_kernel(
     const GGroup* __restrict__ groups, const GTile* __restrict__ tiles,
     const int64_t* __restrict__ colB, const int64_t* __restrict__ colC, const int64_t* __restrict__ colA,
     const TA* __restrict__ opm, BufTable bt, int tile0) {
     using M = Mfma<R>;
+    constexpr int KS = Lds<R>::KS, SA = Lds<R>::SA, SK = Lds<R>::SK;
+    constexpr int ASZ = (KS * SA > GEMM_TM * SK) ? KS * SA : GEMM_TM * SK;
+    constexpr int NA = GEMM_TM * KS / 256, NB = GEMM_TN * KS / 256;
     const GTile tl = tiles[tile0 + xcd_item(blockIdx.x, gridDim.x)];
     const GGroup g = groups[tl.g];
     const int rhs = blockIdx.y;
     const int m0 = tl.mt * GEMM_TM, n0 = tl.nt * GEMM_TN;
-    const TA* A = g.kb ? opm : opm + g.aoff;
     const V* B = bufp<V>(bt, g.bbuf) + (int64_t)rhs * bt.stride[g.bbuf];
     V* C = bufp<V>(bt, g.cbuf) + (int64_t)rhs * bt.stride[g.cbuf];
-    __shared__ R Ar[GK][LDSP], Ai[GK][LDSP];
-    __shared__ R Br[GK][LDSP], Bi[GK][LDSP];
-    __shared__ int64_t cb[GEMM_TN];
+    __shared__ R Ar[ASZ], Ai[ASZ];
+    __shared__ R Br[GEMM_TN * SK], Bi[GEMM_TN * SK];
+    __shared__ int64_t bo[MAXKB][GEMM_TN];   // B offset of (block, column)
+    __shared__ int64_t ao[MAXKB];            // A offset of each active block
+    __shared__ int used[MAXKB], act[MAXKB], nact_s;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
     const int li = lane & 15, lk = lane >> 4;
-    // contiguous B: cached column offsets; blocked B: validity flag only
-    if (tid < GEMM_TN)
-        cb[tid] = (n0 + tid < g.ncols) ? (g.kb ? 0 : colB[g.bcol0 + n0 + tid]) : -1;
-    const int nkb = g.kb ? (g.K + g.kb - 1) / g.kb : 0;
+    const int kb = g.kb ? g.kb : g.K;
+    const int nkb = g.kb ? (g.K + g.kb - 1) / g.kb : 1;
+    if (tid < MAXKB) used[tid] = 0;
+    __syncthreads();
+    for (int e = tid; e < nkb * GEMM_TN; e += 256) {
+        const int h = e / GEMM_TN, j = e % GEMM_TN;
+        int64_t v = -1;
+        if (n0 + j < g.ncols) v = g.kb ? colB[g.bcol0 + (int64_t)(n0 + j) * nkb + h] : colB[g.bcol0 + n0 + j];
+        bo[h][j] = v;
+        if (v >= 0) used[h] = 1;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int c = 0;
+        for (int h = 0; h < nkb; ++h)
+            if (used[h]) act[c++] = h;
+        nact_s = c;
+    }
+    __syncthreads();
+    const int nact = nact_s;
+    if (tid < nact) ao[tid] = g.kb ? colA[g.acol0 + act[tid]] : g.aoff;
+    const int Keff = g.kb ? nact * kb : g.K;
+    __syncthreads();
+
+    TA ra[NA];
+    V rb[NB];
+    auto load = [&](int k0) {
+        const int blk0 = k0 / kb, kend0 = (blk0 + 1) * kb;
+#pragma unroll
+        for (int l = 0; l < NA; ++l) {
+            const int e = tid + 256 * l;
+            int mm, kk;
+            if (g.transA) { kk = e % KS; mm = e / KS; }
+            else { mm = e % GEMM_TM; kk = e / GEMM_TM; }
+            const int m = m0 + mm, k = k0 + kk;
+            TA a;
+            a.x = 0;
+            a.y = 0;
+            if (m < g.M && k < Keff) {
+                const int h = blk0 + (k >= kend0);
+                const TA* Ab = opm + ao[h];
+                const int kl = k - h * kb;
+                a = g.transA ? Ab[kl + (int64_t)m * g.lda] : Ab[m + (int64_t)kl * g.lda];
+            }
+            ra[l] = a;
+        }
+#pragma unroll
+        for (int l = 0; l < NB; ++l) {
+            const int e = tid + 256 * l;
+            const int kk = e % KS, nn = e / KS;
+            const int k = k0 + kk;
+            V b = mk<V>(0, 0);
+            if (k < Keff) {
+                const int h = blk0 + (k >= kend0);
+                const int64_t c = bo[g.kb ? act[h] : 0][nn];
+                if (c >= 0) b = B[c + (k - h * kb)];
+            }
+            rb[l] = b;
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int l = 0; l < NA; ++l) {
+            const int e = tid + 256 * l;
+            int idx;
+            if (g.transA) idx = (e / KS) * SK + (e % KS);           // [m][k]
+            else idx = (e / GEMM_TM) * SA + (e % GEMM_TM);          // [k][m]
+            Ar[idx] = (R)ra[l].x;
+            Ai[idx] = (R)ra[l].y;
+        }
+#pragma unroll
+        for (int l = 0; l < NB; ++l) {
+            const int e = tid + 256 * l;
+            const int idx = (e / KS) * SK + (e % KS);                // [n][k]
+            Br[idx] = (R)rb[l].x;
+            Bi[idx] = (R)rb[l].y;
+        }
+    };
+
     typename M::acc4 cr[2][2], ci[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -252,67 +340,30 @@ __global__ __launch_bounds__(256) void gemm_kernel(
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int v = 0; v < 4; ++v) { cr[a][b][v] = 0; ci[a][b][v] = 0; }
-    for (int k0 = 0; k0 < g.K; k0 += GK) {
-        __syncthreads();
+
+    if (Keff > 0) {
+        load(0);
+        store();
+    }
+    __syncthreads();
+    for (int k0 = 0; k0 < Keff; k0 += KS) {
+        const bool more = k0 + KS < Keff;
+        if (more) load(k0 + KS);          // global loads in flight during the MFMAs below
 #pragma unroll
-        for (int l = 0; l < (GEMM_TM * GK) / 256; ++l) {
-            const int e = tid + 256 * l;
-            int mm, kk;
-            if (g.transA) { kk = e % GK; mm = e / GK; }
-            else { mm = e % GEMM_TM; kk = e / GEMM_TM; }
-            const int m = m0 + mm, k = k0 + kk;
-            R re = 0, im = 0;
-            if (m < g.M && k < g.K) {
-                const TA* Ab = A;
-                int kl = k;
-                if (g.kb) {                 // A column block of this k
-                    const int blk = k / g.kb;
-                    Ab = opm + colA[g.acol0 + blk];
-                    kl = k - blk * g.kb;
-                }
-                const TA a = g.transA ? Ab[kl + (int64_t)m * g.lda] : Ab[m + (int64_t)kl * g.lda];
-                re = (R)a.x;
-                im = (R)a.y;
-            }
-            Ar[kk][mm] = re;
-            Ai[kk][mm] = im;
-        }
-#pragma unroll
-        for (int l = 0; l < (GEMM_TN * GK) / 256; ++l) {
-            const int e = tid + 256 * l;
-            const int kk = e % GK, nn = e / GK;
-            const int k = k0 + kk;
-            const int64_t c = cb[nn];
-            R re = 0, im = 0;
-            if (c >= 0 && k < g.K) {
-                int64_t off = c + k;
-                if (g.kb) {
-                    const int blk = k / g.kb;
-                    const int64_t bo = colB[g.bcol0 + (int64_t)(n0 + nn) * nkb + blk];
-                    off = bo + (k - blk * g.kb);
-                }
-                if (off >= 0) {
-                    const V b = B[off];
-                    re = (R)b.x;
-                    im = (R)b.y;
-                }
-            }
-            Br[kk][nn] = re;
-            Bi[kk][nn] = im;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < GK; kk += 4) {
+        for (int kk = 0; kk < KS; kk += 4) {
             R ar[2], ai[2], br[2], bi[2];
 #pragma unroll
             for (int a = 0; a < 2; ++a) {
-                ar[a] = Ar[kk + lk][wm + 16 * a + li];
-                ai[a] = Ai[kk + lk][wm + 16 * a + li];
+                const int m = wm + 16 * a + li, k = kk + lk;
+                const int idx = g.transA ? m * SK + k : k * SA + m;
+                ar[a] = Ar[idx];
+                ai[a] = Ai[idx];
             }
 #pragma unroll
             for (int b = 0; b < 2; ++b) {
-                br[b] = Br[kk + lk][wn + 16 * b + li];
-                bi[b] = Bi[kk + lk][wn + 16 * b + li];
+                const int idx = (wn + 16 * b + li) * SK + kk + lk;
+                br[b] = Br[idx];
+                bi[b] = Bi[idx];
             }
 #pragma unroll
             for (int a = 0; a < 2; ++a)
@@ -323,6 +374,11 @@ __global__ __launch_bounds__(256) void gemm_kernel(
                     ci[a][b] = M::op(ar[a], bi[b], ci[a][b]);
                     ci[a][b] = M::op(ai[a], br[b], ci[a][b]);
                 }
+        }
+        __syncthreads();
+        if (more) {
+            store();
+            __syncthreads();
         }
     }
     // D layout of a 16x16 MFMA tile: lane holds rows M::row(lane/16, v), column lane%16

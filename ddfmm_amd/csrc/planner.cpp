@@ -98,6 +98,8 @@ struct Builder {
     int add_group(GGroup g, const std::vector<int64_t>& bc, const std::vector<int64_t>& cc,
                   double* flops, int kb = 0, const std::vector<int64_t>& ablk = {}) {
         if (cc.empty()) return -1;
+        if (kb > 0 && (kb < 32 || (g.K + kb - 1) / kb > 8))
+            throw Error("internal: K blocks must hold at least 32 rows, at most 8 blocks");
         g.kb = kb;
         g.acol0 = (int)pl.colA.size();
         pl.colA.insert(pl.colA.end(), ablk.begin(), ablk.end());
@@ -108,11 +110,18 @@ struct Builder {
         pl.colB.insert(pl.colB.end(), bc.begin(), bc.end());
         int gi = (int)pl.groups.size();
         pl.groups.push_back(g);
+        pl.group_flops.push_back(0);
         int mt = (g.M + GEMM_TM - 1) / GEMM_TM, nt = (g.ncols + GEMM_TN - 1) / GEMM_TN;
         for (int a = 0; a < mt; ++a)
             for (int b = 0; b < nt; ++b) pl.tiles.push_back(GTile{gi, a, b, 0});
         double f = 8.0 * g.M * g.K * g.ncols;
+        if (kb > 0) {                        // useful work: present blocks only
+            int64_t present = 0;
+            for (int64_t v : bc) present += v >= 0;
+            f = 8.0 * g.M * kb * (double)present;
+        }
         pl.stats.gemm_flops += f;
+        pl.group_flops[gi] = f;
         if (flops) *flops += f;
         return gi;
     }
@@ -157,7 +166,7 @@ struct HfDir {          // one (width, direction) group
     int64_t uep_g, ucp_g, uepc_g;   // generator point offsets (shuffled)
     int64_t ue2uc[8];   // generated matrices (relative offsets)
     int64_t de2dc[8];   // W = 1 only: G(uep_LF(W/2) + s_c, -ucp) (not a transpose of ue2uc)
-    int64_t e1;         // E1 * E2 * E3, r1 x r2
+    int64_t e1, e2, e3; // fused: e1 = E1 E2 E3 (r1 x r2); factored: E1, E2, E3
     std::vector<BoxKey> out, in;
 };
 
@@ -214,7 +223,7 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
         const LowFreqEntry* le;
         int r;
         int64_t uep_p, ucp_p;          // kapply point offsets
-        int64_t pinv, tab;             // opmat offsets
+        int64_t pinv, V, invs, UH, tab;   // opmat offsets (fused solve or its factors)
         int64_t ue2uc[8];              // generated
         int grid;
         bool has_children;
@@ -231,12 +240,17 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
         w.r = le.uc2ue[1].m;
         w.uep_p = add_pts(le.uep);
         w.ucp_p = add_pts(le.ucp);
-        {
+        if (prm.fuse_solves) {
             // check -> equivalent solve V * diag(1/s) * U^H as one FP64 matrix (NL x NL)
             CMat Vs = le.uc2ue[0];
             for (int j = 0; j < Vs.n; ++j)
                 for (int i = 0; i < Vs.m; ++i) Vs.a[i + (size_t)j * Vs.m] *= le.uc2ue[1].a[j];
             w.pinv = upload(matmul(Vs, le.uc2ue[2]));
+        } else {
+            // factored as the reference applies it (EvalUpwardLow, wave3d_eval.cpp:467-478)
+            w.V = upload(le.uc2ue[0]);
+            w.invs = upload(le.uc2ue[1]);
+            w.UH = upload(le.uc2ue[2]);
         }
         w.tab = (int64_t)pl.opmat.size();
         pl.opmat.resize(pl.opmat.size() + (size_t)343 * G3, cpx(0, 0));
@@ -314,11 +328,20 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
             auto key = std::make_pair(W, h.srt);
             auto it = hf_entry_off.find(key);
             if (it == hf_entry_off.end()) {
-                // E1 * E2 * E3 (r1 x r2) as one FP64 matrix
-                int64_t e = upload(matmul(matmul(h.he->uc2ue[0], h.he->uc2ue[1]), h.he->uc2ue[2]));
-                it = hf_entry_off.emplace(key, std::make_pair(e, e)).first;
+                int64_t e1, e2;
+                if (prm.fuse_solves) {
+                    // E1 * E2 * E3 (r1 x r2) as one FP64 matrix
+                    e1 = e2 = upload(matmul(matmul(h.he->uc2ue[0], h.he->uc2ue[1]), h.he->uc2ue[2]));
+                } else {
+                    e1 = upload(h.he->uc2ue[0]);
+                    e2 = upload(h.he->uc2ue[1]);
+                    upload(h.he->uc2ue[2]);
+                }
+                it = hf_entry_off.emplace(key, std::make_pair(e1, e2)).first;
             }
             h.e1 = it->second.first;
+            h.e2 = it->second.second;
+            h.e3 = h.e2 + (int64_t)h.r1 * h.r2;
             // shuffled point sets and child kernels (UpwardHighFetch, mlib3d.cpp:176-224)
             std::vector<double> uep = Mlib::shuffled(h.prm, h.sgn, h.he->uep);
             std::vector<double> ucp = Mlib::shuffled(h.prm, h.sgn, h.he->ucp);
@@ -408,12 +431,12 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
     pl.bufsize[BUF_VAL] = N;
     pl.bufsize[BUF_LF_UPEQ] = nlf * NL;
     pl.bufsize[BUF_LF_CHK] = nlf * NL;
-    pl.bufsize[BUF_LF_TMP] = 0;                 // unused since the solves are single matrices
+    pl.bufsize[BUF_LF_TMP] = prm.fuse_solves ? 0 : nlf * NL;
     pl.bufsize[BUF_LF_DNEQ] = nlf * NL;
     pl.bufsize[BUF_HF_UP] = hf_up_size;
     pl.bufsize[BUF_HF_DN] = hf_dn_size;
     pl.bufsize[BUF_HF_A] = scr_size;
-    pl.bufsize[BUF_HF_B] = 0;
+    pl.bufsize[BUF_HF_B] = prm.fuse_solves ? 0 : scr_size;
 
     // ================= stages
     stage(ST_GATHER, 0, 0, "gather");
@@ -442,42 +465,54 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
         stage(ST_KAPPLY, t0, (int)pl.ktasks.size(), "lf_s2m");
         int g0 = (int)pl.tiles.size();
         if (w.has_children) {
-            // one GEMM per child-occupancy mask: the present children's kernels side by side
-            // (NL x |mask| NL) times their equivalent densities stacked along K
-            std::map<int, std::pair<std::vector<int64_t>, std::vector<int64_t>>> bymask;
+            // one GEMM: [ue2uc_0 .. ue2uc_7] (NL x 8 NL) times the children stacked along K;
+            // columns sorted by child-occupancy mask so that a tile's union of present
+            // children (the K blocks it multiplies) stays small
+            std::vector<std::pair<int, BoxKey>> cols;
             for (auto& k : kv.second) {
                 if (boxes.at(k).terminal()) continue;
                 int mask = 0;
-                std::vector<int64_t> ch;
-                for (int c = 0; c < 8; ++c) {
-                    auto it = lfidx.find(Tree::child_key(k, c));
-                    if (it == lfidx.end()) continue;
-                    mask |= 1 << c;
-                    ch.push_back(it->second * NL);
-                }
-                auto& e = bymask[mask];
-                e.first.insert(e.first.end(), ch.begin(), ch.end());
-                e.second.push_back(lslot(k));
-            }
-            for (auto& mk : bymask) {
-                std::vector<int64_t> ab;
                 for (int c = 0; c < 8; ++c)
-                    if (mk.first & (1 << c)) ab.push_back(G_(w.ue2uc[c]));
-                GGroup g{};
-                g.aoff = -1; g.soff = -1; g.M = NL; g.K = (int)ab.size() * NL; g.lda = NL; g.transA = 0;
-                g.bbuf = BUF_LF_UPEQ; g.cbuf = BUF_LF_CHK; g.mode = 0;
-                add_group(g, mk.second.first, mk.second.second, nullptr, NL, ab);
+                    if (lfidx.count(Tree::child_key(k, c))) mask |= 1 << c;
+                cols.push_back({mask, k});
             }
+            std::stable_sort(cols.begin(), cols.end(),
+                             [](const std::pair<int, BoxKey>& a, const std::pair<int, BoxKey>& b) { return a.first < b.first; });
+            std::vector<int64_t> bb, cc, ab;
+            for (auto& mk : cols) {
+                for (int c = 0; c < 8; ++c) {
+                    auto it = lfidx.find(Tree::child_key(mk.second, c));
+                    bb.push_back(it == lfidx.end() ? -1 : it->second * NL);
+                }
+                cc.push_back(lslot(mk.second));
+            }
+            for (int c = 0; c < 8; ++c) ab.push_back(G_(w.ue2uc[c]));
+            GGroup g{};
+            g.aoff = -1; g.soff = -1; g.M = NL; g.K = 8 * NL; g.lda = NL; g.transA = 0;
+            g.bbuf = BUF_LF_UPEQ; g.cbuf = BUF_LF_CHK; g.mode = 0;
+            add_group(g, bb, cc, nullptr, NL, ab);
         }
         stage(ST_GEMM, g0, (int)pl.tiles.size(), "lf_m2m");
         std::vector<int64_t> all;
         for (auto& k : kv.second) all.push_back(lslot(k));
         g0 = (int)pl.tiles.size();
         GGroup g{};
-        g.aoff = w.pinv; g.soff = -1; g.M = NL; g.K = NL; g.lda = NL; g.transA = 0;
-        g.bbuf = BUF_LF_CHK; g.cbuf = BUF_LF_UPEQ; g.mode = 0;
-        add_group(g, all, all, nullptr);
-        stage(ST_GEMM, g0, (int)pl.tiles.size(), "lf_uc2ue");
+        if (prm.fuse_solves) {
+            g.aoff = w.pinv; g.soff = -1; g.M = NL; g.K = NL; g.lda = NL; g.transA = 0;
+            g.bbuf = BUF_LF_CHK; g.cbuf = BUF_LF_UPEQ; g.mode = 0;
+            add_group(g, all, all, nullptr);
+            stage(ST_GEMM, g0, (int)pl.tiles.size(), "lf_uc2ue");
+        } else {
+            g.aoff = w.UH; g.soff = w.invs; g.M = w.r; g.K = NL; g.lda = w.r; g.transA = 0;
+            g.bbuf = BUF_LF_CHK; g.cbuf = BUF_LF_TMP; g.mode = 0;
+            add_group(g, all, all, nullptr);
+            stage(ST_GEMM, g0, (int)pl.tiles.size(), "lf_uc2ue_1");
+            g0 = (int)pl.tiles.size();
+            g.aoff = w.V; g.soff = -1; g.M = NL; g.K = w.r; g.lda = NL;
+            g.bbuf = BUF_LF_TMP; g.cbuf = BUF_LF_UPEQ;
+            add_group(g, all, all, nullptr);
+            stage(ST_GEMM, g0, (int)pl.tiles.size(), "lf_uc2ue_2");
+        }
     }
     stage(ST_ZERO, 0, 0, "zero_lfchk", BUF_LF_CHK, pl.bufsize[BUF_LF_CHK]);
 
@@ -487,47 +522,70 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
         int g0 = (int)pl.tiles.size();
         for (auto& h : lv.second) {
             Index3 pd = W == 1.0 ? Index3() : parent_dir(h.dir);
-            // one GEMM per (direction, child-occupancy mask): present children's kernels
-            // side by side (r2 x |mask| r1c), their equivalent densities stacked along K
-            std::map<int, std::pair<std::vector<int64_t>, std::vector<int64_t>>> bymask;
+            // one GEMM per direction: [ue2uc_0 .. ue2uc_7] (r2 x 8 r1c), children stacked
+            // along K, columns sorted by child-occupancy mask (see the LF M2M above)
+            std::vector<std::pair<int, BoxKey>> cols;
             for (auto& k : h.out) {
                 int mask = 0;
-                std::vector<int64_t> ch;
                 for (int c = 0; c < 8; ++c) {
-                    BoxKey ck = Tree::child_key(k, c);
-                    auto bi = boxes.find(ck);
-                    if (bi == boxes.end() || !bi->second.has_pts()) continue;
-                    mask |= 1 << c;
-                    ch.push_back(W == 1.0 ? lslot(ck) : outslot.at({ck, pd}));
+                    auto bi = boxes.find(Tree::child_key(k, c));
+                    if (bi != boxes.end() && bi->second.has_pts()) mask |= 1 << c;
                 }
-                auto& e = bymask[mask];
-                e.first.insert(e.first.end(), ch.begin(), ch.end());
-                e.second.push_back(oscr.at({k, h.dir}));
+                cols.push_back({mask, k});
             }
-            for (auto& mk : bymask) {
-                std::vector<int64_t> ab;
-                for (int c = 0; c < 8; ++c)
-                    if (mk.first & (1 << c)) ab.push_back(G_(h.ue2uc[c]));
-                GGroup g{};
-                g.aoff = -1; g.soff = -1; g.M = h.r2; g.K = (int)ab.size() * h.r1c; g.lda = h.r2; g.transA = 0;
-                g.bbuf = W == 1.0 ? BUF_LF_UPEQ : BUF_HF_UP; g.cbuf = BUF_HF_A; g.mode = 0;
-                add_group(g, mk.second.first, mk.second.second, nullptr, h.r1c, ab);
+            std::stable_sort(cols.begin(), cols.end(),
+                             [](const std::pair<int, BoxKey>& a, const std::pair<int, BoxKey>& b) { return a.first < b.first; });
+            std::vector<int64_t> bb, cc, ab;
+            for (auto& mk : cols) {
+                for (int c = 0; c < 8; ++c) {
+                    if (!(mk.first & (1 << c))) { bb.push_back(-1); continue; }
+                    BoxKey ck = Tree::child_key(mk.second, c);
+                    bb.push_back(W == 1.0 ? lslot(ck) : outslot.at({ck, pd}));
+                }
+                cc.push_back(oscr.at({mk.second, h.dir}));
             }
+            for (int c = 0; c < 8; ++c) ab.push_back(G_(h.ue2uc[c]));
+            GGroup g{};
+            g.aoff = -1; g.soff = -1; g.M = h.r2; g.K = 8 * h.r1c; g.lda = h.r2; g.transA = 0;
+            g.bbuf = W == 1.0 ? BUF_LF_UPEQ : BUF_HF_UP; g.cbuf = BUF_HF_A; g.mode = 0;
+            add_group(g, bb, cc, nullptr, h.r1c, ab);
         }
         stage(ST_GEMM, g0, (int)pl.tiles.size(), "hf_u2u");
-        g0 = (int)pl.tiles.size();
-        for (auto& h : lv.second) {
-            std::vector<int64_t> bc, cc;
-            for (auto& k : h.out) {
-                bc.push_back(oscr.at({k, h.dir}));
-                cc.push_back(outslot.at({k, h.dir}));
+        if (prm.fuse_solves) {
+            g0 = (int)pl.tiles.size();
+            for (auto& h : lv.second) {
+                std::vector<int64_t> bc, cc;
+                for (auto& k : h.out) {
+                    bc.push_back(oscr.at({k, h.dir}));
+                    cc.push_back(outslot.at({k, h.dir}));
+                }
+                GGroup g{};
+                g.aoff = h.e1; g.soff = -1; g.M = h.r1; g.K = h.r2; g.lda = h.r1; g.transA = 0; g.mode = 0;
+                g.bbuf = BUF_HF_A; g.cbuf = BUF_HF_UP;
+                add_group(g, bc, cc, nullptr);
             }
-            GGroup g{};
-            g.aoff = h.e1; g.soff = -1; g.M = h.r1; g.K = h.r2; g.lda = h.r1; g.transA = 0; g.mode = 0;
-            g.bbuf = BUF_HF_A; g.cbuf = BUF_HF_UP;
-            add_group(g, bc, cc, nullptr);
+            stage(ST_GEMM, g0, (int)pl.tiles.size(), "hf_up_e");
+        } else {
+            // E3, then E2, then E1 (EvalUpwardHigh, wave3d_eval.cpp:833-852)
+            const char* nm[3] = {"hf_up_e3", "hf_up_e2", "hf_up_e1"};
+            for (int step = 0; step < 3; ++step) {
+                g0 = (int)pl.tiles.size();
+                for (auto& h : lv.second) {
+                    std::vector<int64_t> bc, cc;
+                    for (auto& k : h.out) {
+                        bc.push_back(oscr.at({k, h.dir}));
+                        cc.push_back(step == 2 ? outslot.at({k, h.dir}) : oscr.at({k, h.dir}));
+                    }
+                    GGroup g{};
+                    g.soff = -1; g.transA = 0; g.mode = 0;
+                    if (step == 0) { g.aoff = h.e3; g.M = h.r2; g.K = h.r2; g.lda = h.r2; g.bbuf = BUF_HF_A; g.cbuf = BUF_HF_B; }
+                    if (step == 1) { g.aoff = h.e2; g.M = h.r1; g.K = h.r2; g.lda = h.r1; g.bbuf = BUF_HF_B; g.cbuf = BUF_HF_A; }
+                    if (step == 2) { g.aoff = h.e1; g.M = h.r1; g.K = h.r1; g.lda = h.r1; g.bbuf = BUF_HF_A; g.cbuf = BUF_HF_UP; }
+                    add_group(g, bc, cc, nullptr);
+                }
+                stage(ST_GEMM, g0, (int)pl.tiles.size(), nm[step]);
+            }
         }
-        stage(ST_GEMM, g0, (int)pl.tiles.size(), "hf_up_e");
     }
 
     // ---- HF downward (W = max .. 1), EvalDownwardHigh 996-1040
@@ -544,20 +602,42 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
             pl.stats.m2l_groups++;
         }
         stage(ST_GEMM, g0, (int)pl.tiles.size(), "hf_m2l");
-        g0 = (int)pl.tiles.size();
-        for (auto& h : lt->second) {
-            std::vector<int64_t> bc, cc;
-            for (auto& k : h.in) {
-                bc.push_back(inslot.at({k, h.dir}));
-                cc.push_back(iscr.at({k, h.dir}));
+        if (prm.fuse_solves) {
+            g0 = (int)pl.tiles.size();
+            for (auto& h : lt->second) {
+                std::vector<int64_t> bc, cc;
+                for (auto& k : h.in) {
+                    bc.push_back(inslot.at({k, h.dir}));
+                    cc.push_back(iscr.at({k, h.dir}));
+                }
+                // (E1 E2 E3)^T = E3^T E2^T E1^T (DownwardHighFetch transposes, mlib3d.cpp:247-252)
+                GGroup g{};
+                g.aoff = h.e1; g.soff = -1; g.M = h.r2; g.K = h.r1; g.lda = h.r1; g.transA = 1; g.mode = 0;
+                g.bbuf = BUF_HF_DN; g.cbuf = BUF_HF_A;
+                add_group(g, bc, cc, nullptr);
             }
-            // (E1 E2 E3)^T = E3^T E2^T E1^T (DownwardHighFetch transposes, mlib3d.cpp:247-252)
-            GGroup g{};
-            g.aoff = h.e1; g.soff = -1; g.M = h.r2; g.K = h.r1; g.lda = h.r1; g.transA = 1; g.mode = 0;
-            g.bbuf = BUF_HF_DN; g.cbuf = BUF_HF_A;
-            add_group(g, bc, cc, nullptr);
+            stage(ST_GEMM, g0, (int)pl.tiles.size(), "hf_dn_et");
+        } else {
+            // E1^T, then E2^T, then E3^T (HighFrequencyL2L, wave3d_eval.cpp:941-949)
+            const char* nm[3] = {"hf_dn_e1t", "hf_dn_e2t", "hf_dn_e3t"};
+            for (int step = 0; step < 3; ++step) {
+                g0 = (int)pl.tiles.size();
+                for (auto& h : lt->second) {
+                    std::vector<int64_t> bc, cc;
+                    for (auto& k : h.in) {
+                        bc.push_back(step == 0 ? inslot.at({k, h.dir}) : iscr.at({k, h.dir}));
+                        cc.push_back(iscr.at({k, h.dir}));
+                    }
+                    GGroup g{};
+                    g.soff = -1; g.transA = 1; g.mode = 0;
+                    if (step == 0) { g.aoff = h.e1; g.M = h.r1; g.K = h.r1; g.lda = h.r1; g.bbuf = BUF_HF_DN; g.cbuf = BUF_HF_A; }
+                    if (step == 1) { g.aoff = h.e2; g.M = h.r2; g.K = h.r1; g.lda = h.r1; g.bbuf = BUF_HF_A; g.cbuf = BUF_HF_B; }
+                    if (step == 2) { g.aoff = h.e3; g.M = h.r2; g.K = h.r2; g.lda = h.r2; g.bbuf = BUF_HF_B; g.cbuf = BUF_HF_A; }
+                    add_group(g, bc, cc, nullptr);
+                }
+                stage(ST_GEMM, g0, (int)pl.tiles.size(), nm[step]);
+            }
         }
-        stage(ST_GEMM, g0, (int)pl.tiles.size(), "hf_dn_et");
         g0 = (int)pl.tiles.size();
         for (auto& h : lt->second) {
             Index3 pd = W == 1.0 ? Index3() : parent_dir(h.dir);
@@ -663,10 +743,23 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
         for (auto& k : lt->second) all.push_back(lslot(k));
         int g0 = (int)pl.tiles.size();
         GGroup g{};
-        g.aoff = w.pinv; g.soff = -1; g.M = NL; g.K = NL; g.lda = NL; g.transA = 1;
-        g.bbuf = BUF_LF_CHK; g.cbuf = BUF_LF_DNEQ; g.mode = 0;
-        add_group(g, all, all, nullptr);
-        stage(ST_GEMM, g0, (int)pl.tiles.size(), "lf_dc2de");
+        if (prm.fuse_solves) {
+            g.aoff = w.pinv; g.soff = -1; g.M = NL; g.K = NL; g.lda = NL; g.transA = 1;
+            g.bbuf = BUF_LF_CHK; g.cbuf = BUF_LF_DNEQ; g.mode = 0;
+            add_group(g, all, all, nullptr);
+            stage(ST_GEMM, g0, (int)pl.tiles.size(), "lf_dc2de");
+        } else {
+            // transpose of the factored solve (DownwardLowFetch, mlib3d.cpp:146-151)
+            g.aoff = w.V; g.soff = w.invs; g.M = w.r; g.K = NL; g.lda = NL; g.transA = 1;
+            g.bbuf = BUF_LF_CHK; g.cbuf = BUF_LF_TMP; g.mode = 0;
+            add_group(g, all, all, nullptr);
+            stage(ST_GEMM, g0, (int)pl.tiles.size(), "lf_dc2de_1");
+            g0 = (int)pl.tiles.size();
+            g.aoff = w.UH; g.soff = -1; g.M = NL; g.K = w.r; g.lda = w.r; g.transA = 1;
+            g.bbuf = BUF_LF_TMP; g.cbuf = BUF_LF_DNEQ;
+            add_group(g, all, all, nullptr);
+            stage(ST_GEMM, g0, (int)pl.tiles.size(), "lf_dc2de_2");
+        }
         if (w.has_children) {
             g0 = (int)pl.tiles.size();
             for (int c = 0; c < 8; ++c) {
@@ -743,8 +836,7 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
                 int gi = pl.tiles[t].g;
                 if (gi == last) continue;
                 last = gi;
-                const GGroup& g = pl.groups[gi];
-                w += 8.0 * g.M * g.K * g.ncols;
+                w += pl.group_flops[gi];
             }
         } else if (s.kind == ST_KAPPLY) {
             for (int t = s.begin; t < s.end; ++t) {

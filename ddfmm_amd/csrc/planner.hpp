@@ -37,6 +37,7 @@ struct HostPlan {
     std::vector<KTask> ktasks;
     std::vector<KSrc> ksrcs;
     std::vector<GGroup> groups;
+    std::vector<double> group_flops;       // useful flops per RHS of each group
     std::vector<GTile> tiles;
     std::vector<int64_t> colB, colC, colA;
     std::vector<FftSrc> fsrc;

@@ -16,8 +16,9 @@ from ._lib import Params, Stats, check, lib, ptr
 
 KERNEL_HELM = 0
 KERNEL_EXPR = 1
-FP64 = 0
-MIXED = 1
+FP64 = 0          # FP64, solves factored as in the reference
+MIXED = 1         # FP32 vectors/translations on MFMA, FP64 fused solves
+FP64_FUSED = 2    # FP64 with fused solves
 
 
 class Kernel3d:

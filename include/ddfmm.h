@@ -21,7 +21,10 @@ typedef struct ddfmm_ctx ddfmm_ctx;
 enum { DDFMM_OK = 0, DDFMM_EINVAL = 1, DDFMM_ESETUP = 2, DDFMM_EDEVICE = 3, DDFMM_ESTATE = 4,
        DDFMM_ENOMEM = 5 };
 enum { DDFMM_KERNEL_HELM = 0, DDFMM_KERNEL_EXPR = 1 };   /* kernel3d.hpp:24-27 */
-enum { DDFMM_FP64 = 0, DDFMM_MIXED = 1 };
+/* FP64: all FP64, check->equivalent solves applied in the reference's factored form.
+   MIXED: FP32 vectors and translations (MFMA f32), FP64 solves as precomputed matrices.
+   FP64_FUSED: FP64 everywhere with the MIXED mode's fused solve matrices. */
+enum { DDFMM_FP64 = 0, DDFMM_MIXED = 1, DDFMM_FP64_FUSED = 2 };
 
 /* Wave3d / Mlib3d / Kernel3d options (wave3d_setup.cpp:29-61, mlib3d.cpp:70-91,
    tt.cpp:103-119): K, ctr, ACCU, NPQ, ptsmax, maxlevel, cell count per edge numC

@@ -24,14 +24,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "hbm": 8.0e12}
-ST_ZERO, ST_GATHER, ST_SCATTER, ST_KAPPLY, ST_GEMM, ST_FFTFWD, ST_VLIST = range(7)
+ST_ZERO, ST_GATHER, ST_SCATTER, ST_KAPPLY, ST_GEMM, ST_FFTFWD, ST_VLIST, ST_REDUCE = range(8)
 
 
 def kernel_class(kind: int, prec: int) -> str:
     if kind == ST_GEMM:
         return "gemm_solve" if prec == 1 else "gemm_translation"
     return {ST_KAPPLY: "kapply", ST_VLIST: "vlist", ST_FFTFWD: "fft_fwd", ST_ZERO: "zero",
-            ST_GATHER: "gather", ST_SCATTER: "scatter"}[kind]
+            ST_GATHER: "gather", ST_SCATTER: "scatter", ST_REDUCE: "reduce"}[kind]
 
 
 def log(msg):
@@ -207,7 +207,8 @@ def main():
     esz = 8 if mixed else 16
     kname = {"gemm_translation": "gemm_kernel<float2,float2,float>" if mixed else "gemm_kernel<double2,double2,double>",
              "gemm_solve": "gemm_kernel<double2,float2,double>" if mixed else "gemm_kernel<double2,double2,double>",
-             "kapply": "kapply_kernel", "vlist": "vlist_kernel<12>", "fft_fwd": "fft_fwd_kernel<12>"}.get(dom, dom)
+             "kapply": "kapply_kernel", "vlist": "vlist_kernel<12>", "fft_fwd": "fft_fwd_kernel<12>",
+             "reduce": "reduce_kernel"}.get(dom, dom)
     if dom in ("gemm_translation", "gemm_solve"):
         peak = PEAK["fp32"] if (mixed and dom == "gemm_translation") else PEAK["fp64"]
         roof = {"bound": "mfma", "achieved": per[dom]["work"] / t_dom / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s"}
@@ -217,6 +218,10 @@ def main():
         peak = PEAK["fp32"] if mixed else PEAK["fp64"]
         roof = {"bound": "mfma", "achieved": work / t_dom / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s"}
         work_desc = "target-source interactions x (20 + 8*nrhs) flops (VALU)"
+    elif dom == "reduce":
+        work = per[dom]["work"] * esz
+        roof = {"bound": "hbm", "achieved": work / t_dom / 1e9, "peak": PEAK["hbm"] / 1e9, "unit": "GB/s"}
+        work_desc = "contribution slots read (targets written not counted)"
     elif dom == "vlist":
         work = ((st["v_sources"] + st["v_targets"]) * 1728 * esz * nrhs + st["nlevels_lf"] * 316 * 1728 * esz)
         roof = {"bound": "hbm", "achieved": work / t_dom / 1e9, "peak": PEAK["hbm"] / 1e9, "unit": "GB/s"}

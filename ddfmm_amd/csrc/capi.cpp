@@ -163,6 +163,7 @@ int ddfmm_setup(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_params* 
         d.groups = c->upload(h.groups); d.tiles = c->upload(h.tiles);
         d.colB = c->upload(h.colB); d.colC = c->upload(h.colC); d.colA = c->upload(h.colA);
         d.fsrc = c->upload(h.fsrc); d.vtrg = c->upload(h.vtrg); d.vpair = c->upload(h.vpair);
+        d.rtrg = c->upload(h.rtrg);
         for (int b = 0; b < BUF_COUNT; ++b) {
             d.bt.stride[b] = h.bufsize[b];
             d.bt.p[b] = esz == 8 ? (void*)c->alloc<float2>(h.bufsize[b] * c->nrhs_max)

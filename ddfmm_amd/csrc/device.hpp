@@ -29,6 +29,7 @@ struct DevPlan {
     FftSrc* fsrc = nullptr;
     VTarget* vtrg = nullptr;
     VPair* vpair = nullptr;
+    RTarget* rtrg = nullptr;
     BufTable bt{};
     double2* den_io = nullptr;     // [rhs][N] input order (gather source)
     double2* val_io = nullptr;     // [rhs][N] input order (scatter target)

@@ -5,8 +5,10 @@
 //                  W/X lists (Kernel3d::kernel + zgemv, kernel3d.cpp:26-107,
 //                  wave3d_eval.cpp:441-451, 550-560, 585-718)
 //   gemm_kernel    grouped complex GEMM on MFMA (16x16x4, f64 or f32) over (box, rhs)
-//                  columns with gather/scatter and store/add/atomic epilogues: every
-//                  translation the reference applies as a per-box zgemv (vecmatop.cpp:102-125)
+//                  columns with gather/scatter and store/add epilogues: every translation
+//                  the reference applies as a per-box zgemv (vecmatop.cpp:102-125)
+//   reduce_kernel  fixed-order sum of the per-contribution slots of the HF M2L/L2L
+//                  targets (the reference's zgemv accumulations into one vector)
 //   fft_fwd_kernel / vlist_kernel   FFT-diagonal LF M2L on the (2P)^3 grid
 //                  (wave3d_eval.cpp:602-661)
 //   kgen_kernel    setup-time translation-matrix generation (mlib3d.cpp:106-280 and the
@@ -211,12 +213,6 @@ template <> struct Mfma<double> {
 
 constexpr int MAXKB = 8;           // at most 8 K blocks (children of a box)
 
-template <class V>
-__device__ __forceinline__ void atomic_add(V* p, typename Real<V>::T x, typename Real<V>::T y) {
-    unsafeAtomicAdd(&p->x, x);
-    unsafeAtomicAdd(&p->y, y);
-}
-
 // LDS layouts (bank-conflict-free operand reads for wave64 MFMA 16x16x4):
 //   A, not transposed: [k][m], row stride SA (f32: 80 = 16 mod 32 words; f64: 68 doubles)
 //   A transposed and B: [m|n][k], row stride SK (f32: KS + 2 = 34, f64: KS + 1 = 17)
@@ -224,7 +220,7 @@ template <class R> struct Lds;
 template <> struct Lds<float> { static constexpr int KS = 32, SA = 80, SK = 34; };
 template <> struct Lds<double> { static constexpr int KS = 16, SA = 68, SK = 17; };
 
-// C[:, ccol] (=|+=|atomic +=) diag(scale) * op(A) * B[:, bcol]; block tile 64 x 64, four
+// C[:, ccol] (=|+=) diag(scale) * op(A) * B[:, bcol]; block tile 64 x 64, four
 // waves of 32 x 32, each a 2 x 2 grid of 16 x 16 MFMA tiles; complex product as four real
 // MFMA products per k-step.  K is staged in slabs of KS through a register-prefetch
 // pipeline: slab s+1 is loaded into registers while slab s runs on the matrix cores.
@@ -403,13 +399,11 @@ __global__ __launch_bounds__(256) void gemm_kernel(
                 }
                 if (g.mode == 0) {
                     Cc[m] = mk<V>(re, im);
-                } else if (g.mode == 1) {
+                } else {
                     V o = Cc[m];
                     o.x += (typename Real<V>::T)re;
                     o.y += (typename Real<V>::T)im;
                     Cc[m] = o;
-                } else {
-                    atomic_add(&Cc[m], (typename Real<V>::T)re, (typename Real<V>::T)im);
                 }
             }
     }
@@ -606,6 +600,34 @@ __global__ void kmatrix_kernel(const double* __restrict__ trg, int M, const doub
     }
 }
 
+// segmented reduction: target t gets the sum of its cnt adjacent contribution slots
+// (M elements each) in a fixed order; one wave per target, four targets per block
+template <class V>
+__global__ __launch_bounds__(256) void reduce_kernel(const RTarget* __restrict__ rtrg, int n, BufTable bt, int t0) {
+    using T = typename Real<V>::T;
+    const int ti = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (ti >= n) return;
+    const RTarget r = rtrg[t0 + ti];
+    const int rhs = blockIdx.y;
+    const V* in = bufp<V>(bt, BUF_RED) + (int64_t)rhs * bt.stride[BUF_RED] + r.red_off;
+    V* out = bufp<V>(bt, r.obuf) + (int64_t)rhs * bt.stride[r.obuf] + r.out_off;
+    for (int m = lane; m < r.M; m += 64) {
+        T sx = 0, sy = 0;
+#pragma unroll 4
+        for (int j = 0; j < r.cnt; ++j) {
+            const V v = in[(int64_t)j * r.M + m];
+            sx += v.x;
+            sy += v.y;
+        }
+        if (r.mode) {
+            const V o = out[m];
+            sx += o.x;
+            sy += o.y;
+        }
+        out[m] = mk<V>(sx, sy);
+    }
+}
+
 int grid_for(int64_t n, int threads) {
     int64_t b = (n + threads - 1) / threads;
     return (int)std::min<int64_t>(std::max<int64_t>(b, 1), 65536);
@@ -661,6 +683,9 @@ hipError_t launch_stage_t(const DevPlan& d, const Stage& s, int nrhs, hipStream_
         else
             vlist_kernel<12, double2><<<dim3(n, nrhs), 256, 0, st>>>(d.vtrg, d.vpair, d.grid, d.opmat, d.NL, d.bt,
                                                                      s.begin);
+        break;
+    case ST_REDUCE:
+        reduce_kernel<V><<<dim3((n + 3) / 4, nrhs), 256, 0, st>>>(d.rtrg, n, d.bt, s.begin);
         break;
     default:
         return hipErrorInvalidValue;

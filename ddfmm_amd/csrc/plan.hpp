@@ -21,6 +21,7 @@ enum Buf : int {
     BUF_HF_A,       // HF per-level scratch
     BUF_HF_B,
     BUF_SPEC,       // LF V-list source spectra of one level, (2P)^3 each
+    BUF_RED,        // HF M2L/L2L contributions of one level, one slot per (target, source)
     BUF_COUNT
 };
 
@@ -46,7 +47,7 @@ struct KTask {
     int pad;
 };
 
-// grouped complex GEMM: C[:, ccol] (=, +=, atomic +=) diag(scale) * op(A) * B[:, bcol]
+// grouped complex GEMM: C[:, ccol] (=, +=) diag(scale) * op(A) * B[:, bcol]
 // B columns: contiguous (kb = 0, offset colB[bcol0 + n]) or blocked along K in blocks of kb
 // rows, each block gathered from its own offset colB[bcol0 + n * K/kb + block], with the
 // matching A column block at colA[acol0 + block] (used to concatenate the non-empty
@@ -57,7 +58,7 @@ struct GGroup {
     int M, K, lda, transA;
     int bbuf, cbuf;
     int ncols, col0;
-    int mode;       // 0 store, 1 add, 2 atomic add
+    int mode;       // 0 store, 1 add
     int kb;         // K block size of the blocked gather (0 = contiguous)
     int bcol0;      // first B offset in colB
     int acol0;      // kb > 0: first A block offset in colA (block b at colA[acol0 + b])
@@ -88,6 +89,15 @@ struct VPair {
     int pad;
 };
 
+// segmented reduction: out[out_off + m] (=|+=) sum_{j < cnt} BUF_RED[red_off + j * M + m]
+// (the contributions of one target are adjacent, in a fixed order: deterministic sums)
+struct RTarget {
+    int64_t out_off;
+    int64_t red_off;
+    int cnt, M;
+    int obuf, mode;     // mode 0 store, 1 add
+};
+
 // setup-time kernel matrix generation: out(i,j) = G((sgn*T_i + tshift) - (S_j + sshift))
 struct KGen {
     int64_t toff, soff, out;   // offsets into the generator point table / operator table
@@ -98,7 +108,7 @@ struct KGen {
 };
 
 enum StageKind : int {
-    ST_ZERO = 0, ST_GATHER, ST_SCATTER, ST_KAPPLY, ST_GEMM, ST_FFTFWD, ST_VLIST
+    ST_ZERO = 0, ST_GATHER, ST_SCATTER, ST_KAPPLY, ST_GEMM, ST_FFTFWD, ST_VLIST, ST_REDUCE
 };
 // Arithmetic class of a stage.  In the mixed-precision mode translation stages
 // (kernel-generated matrices: M2M/L2L, U2U/D2D child kernels, HF M2L, near field, V list)
@@ -112,7 +122,8 @@ struct Stage {
     int64_t count;      // ST_ZERO: elements per rhs
     int prec;           // StagePrec
     int pad;
-    double work;        // per RHS: GEMM useful flops, KAPPLY interactions, FFT spectra, V-list pairs
+    double work;        // per RHS: GEMM useful flops, KAPPLY interactions, FFT spectra, V-list pairs,
+                        // reduction elements read
     char name[24];
 };
 

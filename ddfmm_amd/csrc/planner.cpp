@@ -156,6 +156,93 @@ struct Builder {
     }
 
     void run(const double* pos, int64_t N, bool want_dump);
+
+    // Every descriptor's element range must lie inside the buffer or table it indexes: the
+    // kernels trust these offsets, so a planner mistake is caught here on the host.
+    void validate() const {
+        auto need = [](bool ok, const char* what) {
+            if (!ok) throw Error(std::string("internal: descriptor out of bounds (") + what + ")");
+        };
+        const int64_t npt = (int64_t)pl.px.size();
+        const int64_t G3 = (int64_t)(2 * pl.P) * (2 * pl.P) * (2 * pl.P);
+        for (const Stage& st : pl.stages) {
+            need(st.begin >= 0 && st.begin <= st.end, "stage range");
+            if (st.kind == ST_GEMM) need(st.end <= (int64_t)pl.tiles.size(), "gemm stage");
+            if (st.kind == ST_KAPPLY) need(st.end <= (int64_t)pl.ktasks.size(), "kapply stage");
+            if (st.kind == ST_FFTFWD) need(st.end <= (int64_t)pl.fsrc.size(), "fft stage");
+            if (st.kind == ST_VLIST) need(st.end <= (int64_t)pl.vtrg.size(), "vlist stage");
+            if (st.kind == ST_REDUCE) need(st.end <= (int64_t)pl.rtrg.size(), "reduce stage");
+            if (st.kind == ST_ZERO) need(st.buf >= 0 && st.buf < BUF_COUNT && st.count <= pl.bufsize[st.buf], "zero");
+        }
+        for (const GTile& t : pl.tiles) {
+            need(t.g >= 0 && t.g < (int)pl.groups.size(), "tile group");
+            const GGroup& g = pl.groups[t.g];
+            need(t.mt * GEMM_TM < g.M && t.nt * GEMM_TN < g.ncols, "tile index");
+        }
+        for (const GGroup& g : pl.groups) {
+            need(g.bbuf >= 0 && g.bbuf < BUF_COUNT && g.cbuf >= 0 && g.cbuf < BUF_COUNT, "gemm buffers");
+            need(g.col0 >= 0 && g.col0 + (int64_t)g.ncols <= (int64_t)pl.colC.size(), "gemm C columns");
+            for (int n = 0; n < g.ncols; ++n) {
+                const int64_t c = pl.colC[g.col0 + n];
+                need(c >= 0 && c + g.M <= pl.bufsize[g.cbuf], "gemm C range");
+            }
+            if (g.soff >= 0) need(g.soff + g.M <= pl.opmat_total, "gemm row scale");
+            auto a_ok = [&](int64_t aoff, int kcols) {
+                const int64_t last = g.transA ? aoff + (int64_t)(g.M - 1) * g.lda + kcols
+                                              : aoff + (int64_t)(kcols - 1) * g.lda + g.M;
+                return aoff >= 0 && last <= pl.opmat_total && g.lda >= (g.transA ? kcols : g.M);
+            };
+            if (g.kb == 0) {
+                need(a_ok(g.aoff, g.K), "gemm A");
+                need(g.bcol0 + (int64_t)g.ncols <= (int64_t)pl.colB.size(), "gemm B columns");
+                for (int n = 0; n < g.ncols; ++n) {
+                    const int64_t b = pl.colB[g.bcol0 + n];
+                    need(b >= 0 && b + g.K <= pl.bufsize[g.bbuf], "gemm B range");
+                }
+            } else {
+                const int nkb = (g.K + g.kb - 1) / g.kb;
+                need(nkb * g.kb == g.K && nkb <= 8, "gemm K blocks");
+                need(g.acol0 + (int64_t)nkb <= (int64_t)pl.colA.size(), "gemm A blocks");
+                for (int h = 0; h < nkb; ++h) need(a_ok(pl.colA[g.acol0 + h], g.kb), "gemm A block");
+                need(g.bcol0 + (int64_t)g.ncols * nkb <= (int64_t)pl.colB.size(), "gemm B blocks");
+                for (int64_t e = 0; e < (int64_t)g.ncols * nkb; ++e) {
+                    const int64_t b = pl.colB[g.bcol0 + e];
+                    need(b >= -1 && b + g.kb <= pl.bufsize[g.bbuf], "gemm B block range");
+                }
+            }
+        }
+        for (const KTask& t : pl.ktasks) {
+            need(t.obuf >= 0 && t.obuf < BUF_COUNT && t.ooff >= 0 && t.ooff + t.count <= pl.bufsize[t.obuf], "kapply out");
+            need(t.off >= 0 && t.off + t.count <= npt, "kapply targets");
+            need(t.s0 >= 0 && t.s0 <= t.s1 && t.s1 <= (int)pl.ksrcs.size(), "kapply sources");
+            for (int q = t.s0; q < t.s1; ++q) {
+                const KSrc& k = pl.ksrcs[q];
+                need(k.off >= 0 && k.off + k.count <= npt, "kapply source points");
+                need(k.vbuf >= 0 && k.vbuf < BUF_COUNT && k.voff >= 0 && k.voff + k.count <= pl.bufsize[k.vbuf],
+                     "kapply source values");
+            }
+        }
+        for (const FftSrc& f : pl.fsrc) {
+            need(f.in_off >= 0 && f.in_off + pl.NL <= pl.bufsize[BUF_LF_UPEQ], "fft input");
+            need(f.spec_off >= 0 && f.spec_off + G3 <= pl.bufsize[BUF_SPEC], "fft spectrum");
+            need(f.grid >= 0 && f.grid + pl.NL <= (int64_t)pl.grid.size(), "fft grid");
+        }
+        for (const VTarget& v : pl.vtrg) {
+            need(v.out_off >= 0 && v.out_off + pl.NL <= pl.bufsize[BUF_LF_CHK], "vlist out");
+            need(v.tab_off >= 0 && v.tab_off + 343 * G3 <= pl.opmat_total, "vlist table");
+            need(v.grid >= 0 && v.grid + pl.NL <= (int64_t)pl.grid.size(), "vlist grid");
+            need(v.p0 >= 0 && v.p0 <= v.p1 && v.p1 <= (int)pl.vpair.size(), "vlist pairs");
+            for (int q = v.p0; q < v.p1; ++q) {
+                need(pl.vpair[q].spec_off >= 0 && pl.vpair[q].spec_off + G3 <= pl.bufsize[BUF_SPEC], "vlist source");
+                need(pl.vpair[q].offidx >= 0 && pl.vpair[q].offidx < 343, "vlist offset");
+            }
+        }
+        for (const RTarget& r : pl.rtrg) {
+            need(r.obuf >= 0 && r.obuf < BUF_COUNT && r.out_off >= 0 && r.out_off + r.M <= pl.bufsize[r.obuf],
+                 "reduce out");
+            need(r.cnt >= 0 && r.red_off >= 0 && r.red_off + (int64_t)r.cnt * r.M <= pl.bufsize[BUF_RED], "reduce in");
+        }
+    }
 };
 
 struct HfDir {          // one (width, direction) group
@@ -379,11 +466,71 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
         scr_size = std::max(scr_size, std::max(lvl_o, lvl_i));
     }
 
+    // ---------------- reduction targets of the HF downward pass
+    // Every in-slot (box, direction) of a level receives its M2L contributions (one per
+    // far-field source box) and its L2L contributions (one per direction of the parent that
+    // refines this direction); the LF children of width-1 boxes receive one L2L contribution
+    // per incoming direction of the parent.  Each contribution is written by a plain store
+    // into its own slot of BUF_RED (the slots of one target adjacent), then one reduction per
+    // level sums them into the target: no atomics, fixed summation order.
+    struct RedT {
+        int64_t out_off;
+        int obuf, M, mode;
+        int cnt = 0, next = 0;
+        int64_t base = 0;
+    };
+    std::vector<RedT> rts;
+    std::map<double, std::pair<int, int>> hf_region;          // W -> target range of the level
+    std::map<std::pair<BoxKey, Index3>, int> hf_rt;           // in-slot -> target
+    std::map<BoxKey, int> lf_rt;                              // LF child of a width-1 box -> target
+    for (auto lt = hfl.rbegin(); lt != hfl.rend(); ++lt) {
+        int r0 = (int)rts.size();
+        for (auto& h : lt->second)
+            for (auto& t : h.in) {
+                RedT r;
+                r.out_off = inslot.at({t, h.dir}); r.obuf = BUF_HF_DN; r.M = h.r1; r.mode = 0;
+                hf_rt[{t, h.dir}] = (int)rts.size();
+                rts.push_back(r);
+            }
+        hf_region[lt->first] = {r0, (int)rts.size()};
+    }
+    // L2L contributions, enumerated in the order the L2L groups are emitted below
+    auto l2l_target = [&](const HfDir& h, const BoxKey& ck) -> int {
+        if (h.W == 1.0) {
+            auto it = lf_rt.find(ck);
+            if (it != lf_rt.end()) return it->second;
+            RedT r;
+            r.out_off = lslot(ck); r.obuf = BUF_LF_CHK; r.M = NL; r.mode = 1;
+            lf_rt[ck] = (int)rts.size();
+            rts.push_back(r);
+            return (int)rts.size() - 1;
+        }
+        auto it = hf_rt.find({ck, parent_dir(h.dir)});
+        if (it == hf_rt.end()) throw Error("internal: child lacks inherited direction");
+        return it->second;
+    };
+    auto child_present = [&](const BoxKey& ck) {
+        auto bi = boxes.find(ck);
+        return bi != boxes.end() && bi->second.has_pts();
+    };
+    int lf_r0 = -1;
+    for (auto lt = hfl.rbegin(); lt != hfl.rend(); ++lt) {
+        if (lt->first == 1.0) lf_r0 = (int)rts.size();
+        for (auto& h : lt->second)
+            for (int c = 0; c < 8; ++c)
+                for (auto& k : h.in) {
+                    BoxKey ck = Tree::child_key(k, c);
+                    if (child_present(ck)) rts[l2l_target(h, ck)].cnt++;
+                }
+    }
+    const int lf_r1 = (int)rts.size();
+
     // ---------------- HF M2L pairs grouped by (W, offset) (HighFrequencyM2L, eval 876-928)
     struct M2LGroup {
         const HfDir* h;
         int64_t mat;
-        std::vector<int64_t> b, c;
+        std::vector<int64_t> b;
+        std::vector<int> t;      // reduction target of each pair
     };
     std::map<double, std::map<Index3, M2LGroup>> m2l;
     for (auto& lv : hfl) {
@@ -412,11 +559,43 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
                         throw Error("internal: one M2L offset maps to two directions");
                     }
                     g.b.push_back(os->second);
-                    g.c.push_back(inslot.at({t, h.dir}));
+                    const int ri = hf_rt.at({t, h.dir});
+                    g.t.push_back(ri);
+                    rts[ri].cnt++;
                 }
             }
         }
     }
+
+    // slots: each region (one HF level, or the LF children of width 1) starts at 0
+    int64_t red_size = 0;
+    auto assign = [&](int r0, int r1) {
+        int64_t cur = 0;
+        for (int i = r0; i < r1; ++i) {
+            rts[i].base = cur;
+            cur += (int64_t)rts[i].cnt * rts[i].M;
+        }
+        red_size = std::max(red_size, cur);
+    };
+    for (auto& rg : hf_region) assign(rg.second.first, rg.second.second);
+    if (lf_r0 >= 0) assign(lf_r0, lf_r1);
+    auto slot = [&](int ri) {
+        RedT& r = rts[ri];
+        if (r.next >= r.cnt) throw Error("internal: reduction slot overflow");
+        return r.base + (int64_t)(r.next++) * r.M;
+    };
+    auto reduce_stage = [&](int r0, int r1, const char* name) {
+        int q0 = (int)pl.rtrg.size();
+        for (int i = r0; i < r1; ++i) {
+            const RedT& r = rts[i];
+            if (r.next != r.cnt) throw Error("internal: reduction target missing contributions");
+            if (r.mode == 1 && r.cnt == 0) continue;
+            RTarget t{};
+            t.out_off = r.out_off; t.red_off = r.base; t.cnt = r.cnt; t.M = r.M; t.obuf = r.obuf; t.mode = r.mode;
+            pl.rtrg.push_back(t);
+        }
+        stage(ST_REDUCE, q0, (int)pl.rtrg.size(), name);
+    };
 
     // ---------------- operator table layout: uploaded part, then generated part
     const int64_t up_n = (int64_t)pl.opmat.size();
@@ -437,10 +616,10 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
     pl.bufsize[BUF_HF_DN] = hf_dn_size;
     pl.bufsize[BUF_HF_A] = scr_size;
     pl.bufsize[BUF_HF_B] = prm.fuse_solves ? 0 : scr_size;
+    pl.bufsize[BUF_RED] = red_size;
 
     // ================= stages
     stage(ST_GATHER, 0, 0, "gather");
-    stage(ST_ZERO, 0, 0, "zero_hfdn", BUF_HF_DN, pl.bufsize[BUF_HF_DN]);
 
     // ---- LF upward (W ascending = finest first), EvalUpwardLow 418-490
     for (auto& kv : ldmap) {
@@ -596,12 +775,15 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
             M2LGroup& mg = og.second;
             GGroup g{};
             g.aoff = G_(mg.mat); g.soff = -1; g.M = mg.h->r1; g.K = mg.h->r1; g.lda = mg.h->r1; g.transA = 0;
-            g.bbuf = BUF_HF_UP; g.cbuf = BUF_HF_DN; g.mode = 2;
-            add_group(g, mg.b, mg.c, &pl.stats.m2l_flops);
+            g.bbuf = BUF_HF_UP; g.cbuf = BUF_RED; g.mode = 0;
+            std::vector<int64_t> cc;
+            for (int ri : mg.t) cc.push_back(slot(ri));
+            add_group(g, mg.b, cc, &pl.stats.m2l_flops);
             pl.stats.m2l_pairs += (int64_t)mg.b.size();
             pl.stats.m2l_groups++;
         }
         stage(ST_GEMM, g0, (int)pl.tiles.size(), "hf_m2l");
+        reduce_stage(hf_region.at(W).first, hf_region.at(W).second, "hf_reduce");
         if (prm.fuse_solves) {
             g0 = (int)pl.tiles.size();
             for (auto& h : lt->second) {
@@ -640,20 +822,13 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
         }
         g0 = (int)pl.tiles.size();
         for (auto& h : lt->second) {
-            Index3 pd = W == 1.0 ? Index3() : parent_dir(h.dir);
             for (int c = 0; c < 8; ++c) {
                 std::vector<int64_t> bc, cc;
                 for (auto& k : h.in) {
                     BoxKey ck = Tree::child_key(k, c);
-                    auto bi = boxes.find(ck);
-                    if (bi == boxes.end() || !bi->second.has_pts()) continue;
+                    if (!child_present(ck)) continue;
                     bc.push_back(iscr.at({k, h.dir}));
-                    if (W == 1.0) cc.push_back(lslot(ck));
-                    else {
-                        auto it = inslot.find({ck, pd});
-                        if (it == inslot.end()) throw Error("internal: child lacks inherited direction");
-                        cc.push_back(it->second);
-                    }
+                    cc.push_back(slot(l2l_target(h, ck)));
                 }
                 // W > 1: de2dc(c) = G(-uepc + s_c, -ucp) = ue2uc(7 - c)^T (mlib3d.cpp:227-280);
                 // W = 1: explicit matrix (r1c x r2)
@@ -664,11 +839,12 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
                     g.aoff = G_(h.ue2uc[7 - c]); g.M = h.r1c; g.K = h.r2; g.lda = h.r2; g.transA = 1;
                 }
                 g.soff = -1;
-                g.bbuf = BUF_HF_A; g.cbuf = W == 1.0 ? BUF_LF_CHK : BUF_HF_DN; g.mode = 2;
+                g.bbuf = BUF_HF_A; g.cbuf = BUF_RED; g.mode = 0;
                 add_group(g, bc, cc, nullptr);
             }
         }
         stage(ST_GEMM, g0, (int)pl.tiles.size(), "hf_l2l");
+        if (W == 1.0 && lf_r0 >= 0) reduce_stage(lf_r0, lf_r1, "lf_reduce");
     }
 
     // ---- LF downward (W descending), EvalDownwardLow 493-583
@@ -847,9 +1023,13 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
             w = s.end - s.begin;
         } else if (s.kind == ST_VLIST) {
             for (int t = s.begin; t < s.end; ++t) w += pl.vtrg[t].p1 - pl.vtrg[t].p0;
+        } else if (s.kind == ST_REDUCE) {
+            for (int t = s.begin; t < s.end; ++t) w += (double)pl.rtrg[t].cnt * pl.rtrg[t].M;
         }
         s.work = w;
     }
+
+    validate();
 
     // ---------------- canonical dump of the lists (integer parity with the oracle)
     if (want_dump) {

@@ -43,6 +43,7 @@ struct HostPlan {
     std::vector<FftSrc> fsrc;
     std::vector<VTarget> vtrg;
     std::vector<VPair> vpair;
+    std::vector<RTarget> rtrg;
     std::vector<Stage> stages;
     int64_t bufsize[BUF_COUNT] = {0};      // elements per rhs
     PlanStats stats;

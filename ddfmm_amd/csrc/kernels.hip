@@ -213,86 +213,101 @@ template <> struct Mfma<double> {
 
 constexpr int MAXKB = 8;           // at most 8 K blocks (children of a box)
 
-// LDS layouts (bank-conflict-free operand reads for wave64 MFMA 16x16x4):
-//   A, not transposed: [k][m], row stride SA (f32: 80 = 16 mod 32 words; f64: 68 doubles)
-//   A transposed and B: [m|n][k], row stride SK (f32: KS + 2 = 34, f64: KS + 1 = 17)
+// LDS layouts (conflict-free fragment reads for wave64 MFMA 16x16x4):
+//   A slab [k][m], row stride SA = 16 * GEMM_MB = 112 words = 16 mod 32: the two k rows of
+//   a 32-lane f32 read land in disjoint bank halves (f64 reads serve 16 lanes per cycle);
+//   B slab [n][k], row stride SB: f32 18 (even/odd rows split the banks), f64 17 (odd).
 template <class R> struct Lds;
-template <> struct Lds<float> { static constexpr int KS = 32, SA = 80, SK = 34; };
-template <> struct Lds<double> { static constexpr int KS = 16, SA = 68, SK = 17; };
+template <> struct Lds<float> { static constexpr int SB = 18; };
+template <> struct Lds<double> { static constexpr int SB = 17; };
 
-// C[:, ccol] (=|+=) diag(scale) * op(A) * B[:, bcol]; block tile 64 x 64, four
-// waves of 32 x 32, each a 2 x 2 grid of 16 x 16 MFMA tiles; complex product as four real
-// MFMA products per k-step.  K is staged in slabs of KS through a register-prefetch
-// pipeline: slab s+1 is loaded into registers while slab s runs on the matrix cores.
-// Blocked gathers (kb > 0, kb >= KS): the tile multiplies only the K blocks present in at
-// least one of its columns (compacted list in LDS).
+// C[:, ccol] (=|+=) diag(scale) * op(A) * B[:, bcol] on MFMA 16x16x4 (f32 or f64).
+// A tile covers up to GEMM_MB row blocks of 16 (the group's rows split evenly over its
+// tiles) and GEMM_TN = 64 columns, 16 per wave; complex products as four real MFMAs.
+// K runs over segments (the whole K, or the K blocks present in at least one of the
+// tile's columns) in slabs of 16 with a register-prefetch pipeline: slab s+1 is loaded
+// while slab s runs on the matrix cores.  Padding stays within 16 rows, 4 k and 16
+// columns: the MFMAs stop at the last k step of a segment, and a wave whose columns are
+// all empty, or all lack the current K block, skips the segment's MFMAs.
 template <class TA, class V, class R>
 __global__ __launch_bounds__(256) void gemm_kernel(
     const GGroup* __restrict__ groups, const GTile* __restrict__ tiles,
     const int64_t* __restrict__ colB, const int64_t* __restrict__ colC, const int64_t* __restrict__ colA,
     const TA* __restrict__ opm, BufTable bt, int tile0) {
     using M = Mfma<R>;
-    constexpr int KS = Lds<R>::KS, SA = Lds<R>::SA, SK = Lds<R>::SK;
-    constexpr int ASZ = (KS * SA > GEMM_TM * SK) ? KS * SA : GEMM_TM * SK;
-    constexpr int NA = GEMM_TM * KS / 256, NB = GEMM_TN * KS / 256;
+    using T = typename Real<V>::T;
+    constexpr int KS = 16, MBX = GEMM_MB, TM = 16 * MBX, SA = TM, SB = Lds<R>::SB;
+    constexpr int NA = TM * KS / 256, NB = GEMM_TN * KS / 256;
+    static_assert(NA * 256 == TM * KS && NB * 256 == GEMM_TN * KS, "slab must split evenly over 256 threads");
     const GTile tl = tiles[tile0 + xcd_item(blockIdx.x, gridDim.x)];
     const GGroup g = groups[tl.g];
     const int rhs = blockIdx.y;
-    const int m0 = tl.mt * GEMM_TM, n0 = tl.nt * GEMM_TN;
     const V* B = bufp<V>(bt, g.bbuf) + (int64_t)rhs * bt.stride[g.bbuf];
     V* C = bufp<V>(bt, g.cbuf) + (int64_t)rhs * bt.stride[g.cbuf];
-    __shared__ R Ar[ASZ], Ai[ASZ];
-    __shared__ R Br[GEMM_TN * SK], Bi[GEMM_TN * SK];
-    __shared__ int64_t bo[MAXKB][GEMM_TN];   // B offset of (block, column)
-    __shared__ int64_t ao[MAXKB];            // A offset of each active block
-    __shared__ int used[MAXKB], act[MAXKB], nact_s;
+    __shared__ R Ar[KS * SA], Ai[KS * SA];
+    __shared__ R Br[GEMM_TN * SB], Bi[GEMM_TN * SB];
+    __shared__ int64_t bo[MAXKB][GEMM_TN];   // B offset of (block, column), -1 = absent
+    __shared__ int64_t ao[MAXKB];            // A offset of each active segment
+    __shared__ int used[MAXKB][4], act[MAXKB], wm[MAXKB], nact_s;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
     const int li = lane & 15, lk = lane >> 4;
-    const int kb = g.kb ? g.kb : g.K;
-    const int nkb = g.kb ? (g.K + g.kb - 1) / g.kb : 1;
-    if (tid < MAXKB) used[tid] = 0;
+    const int mrows = min(16 * tl.mb, g.M - tl.m0);     // rows of this tile holding data
+    const int kb = g.kb ? g.kb : g.K;                   // segment length
+    const int nkb = g.kb ? g.K / g.kb : 1;
+    if (tid < MAXKB * 4) used[tid >> 2][tid & 3] = 0;
     __syncthreads();
     for (int e = tid; e < nkb * GEMM_TN; e += 256) {
         const int h = e / GEMM_TN, j = e % GEMM_TN;
         int64_t v = -1;
-        if (n0 + j < g.ncols) v = g.kb ? colB[g.bcol0 + (int64_t)(n0 + j) * nkb + h] : colB[g.bcol0 + n0 + j];
+        if (tl.n0 + j < g.ncols) v = g.kb ? colB[g.bcol0 + (int64_t)(tl.n0 + j) * nkb + h] : colB[g.bcol0 + tl.n0 + j];
         bo[h][j] = v;
-        if (v >= 0) used[h] = 1;
+        if (v >= 0) used[h][j >> 4] = 1;
     }
     __syncthreads();
     if (tid == 0) {
         int c = 0;
-        for (int h = 0; h < nkb; ++h)
-            if (used[h]) act[c++] = h;
+        for (int h = 0; h < nkb; ++h) {
+            const int w = used[h][0] | (used[h][1] << 1) | (used[h][2] << 2) | (used[h][3] << 3);
+            if (w) {
+                act[c] = h;
+                wm[c] = w;
+                ++c;
+            }
+        }
         nact_s = c;
     }
     __syncthreads();
     const int nact = nact_s;
     if (tid < nact) ao[tid] = g.kb ? colA[g.acol0 + act[tid]] : g.aoff;
-    const int Keff = g.kb ? nact * kb : g.K;
     __syncthreads();
 
+    // element e of a slab: A (row, k) and B (k, column)
+    auto a_at = [&](int e, int& mm, int& kk) {
+        if (g.transA) {              // global A[k + m*lda]: 4 consecutive k per row
+            kk = (e & 3) | ((e / (4 * TM)) << 2);
+            mm = (e >> 2) % TM;
+        } else {                     // global A[m + k*lda]: consecutive rows
+            mm = e % TM;
+            kk = e / TM;
+        }
+    };
     TA ra[NA];
     V rb[NB];
-    auto load = [&](int k0) {
-        const int blk0 = k0 / kb, kend0 = (blk0 + 1) * kb;
+    auto load = [&](int i, int k0) {
+        const TA* Ab = opm + ao[i];
+        const int hb = g.kb ? act[i] : 0;
+        const int kl = kb - k0;
 #pragma unroll
         for (int l = 0; l < NA; ++l) {
-            const int e = tid + 256 * l;
             int mm, kk;
-            if (g.transA) { kk = e % KS; mm = e / KS; }
-            else { mm = e % GEMM_TM; kk = e / GEMM_TM; }
-            const int m = m0 + mm, k = k0 + kk;
+            a_at(tid + 256 * l, mm, kk);
             TA a;
             a.x = 0;
             a.y = 0;
-            if (m < g.M && k < Keff) {
-                const int h = blk0 + (k >= kend0);
-                const TA* Ab = opm + ao[h];
-                const int kl = k - h * kb;
-                a = g.transA ? Ab[kl + (int64_t)m * g.lda] : Ab[m + (int64_t)kl * g.lda];
+            if (mm < mrows && kk < kl) {
+                const int m = tl.m0 + mm, k = k0 + kk;
+                a = g.transA ? Ab[k + (int64_t)m * g.lda] : Ab[m + (int64_t)k * g.lda];
             }
             ra[l] = a;
         }
@@ -300,112 +315,109 @@ __global__ __launch_bounds__(256) void gemm_kernel(
         for (int l = 0; l < NB; ++l) {
             const int e = tid + 256 * l;
             const int kk = e % KS, nn = e / KS;
-            const int k = k0 + kk;
             V b = mk<V>(0, 0);
-            if (k < Keff) {
-                const int h = blk0 + (k >= kend0);
-                const int64_t c = bo[g.kb ? act[h] : 0][nn];
-                if (c >= 0) b = B[c + (k - h * kb)];
-            }
+            const int64_t c = bo[hb][nn];
+            if (c >= 0 && kk < kl) b = B[c + k0 + kk];
             rb[l] = b;
         }
     };
     auto store = [&]() {
 #pragma unroll
         for (int l = 0; l < NA; ++l) {
-            const int e = tid + 256 * l;
-            int idx;
-            if (g.transA) idx = (e / KS) * SK + (e % KS);           // [m][k]
-            else idx = (e / GEMM_TM) * SA + (e % GEMM_TM);          // [k][m]
-            Ar[idx] = (R)ra[l].x;
-            Ai[idx] = (R)ra[l].y;
+            int mm, kk;
+            a_at(tid + 256 * l, mm, kk);
+            if (mm < 16 * tl.mb) {
+                Ar[kk * SA + mm] = (R)ra[l].x;
+                Ai[kk * SA + mm] = (R)ra[l].y;
+            }
         }
 #pragma unroll
         for (int l = 0; l < NB; ++l) {
             const int e = tid + 256 * l;
-            const int idx = (e / KS) * SK + (e % KS);                // [n][k]
+            const int idx = (e / KS) * SB + (e % KS);
             Br[idx] = (R)rb[l].x;
             Bi[idx] = (R)rb[l].y;
         }
     };
 
-    typename M::acc4 cr[2][2], ci[2][2];
+    typename M::acc4 cr[MBX], ci[MBX];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < MBX; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) { cr[a][b][v] = 0; ci[a][b][v] = 0; }
+        for (int v = 0; v < 4; ++v) {
+            cr[a][v] = 0;
+            ci[a][v] = 0;
+        }
 
-    if (Keff > 0) {
-        load(0);
+    if (nact > 0) {
+        load(0, 0);
         store();
     }
     __syncthreads();
-    for (int k0 = 0; k0 < Keff; k0 += KS) {
-        const bool more = k0 + KS < Keff;
-        if (more) load(k0 + KS);          // global loads in flight during the MFMAs below
+    int i = 0, k0 = 0;
+    while (i < nact) {
+        int ni = i, nk0 = k0 + KS;
+        if (nk0 >= kb) {
+            ni = i + 1;
+            nk0 = 0;
+        }
+        const bool more = ni < nact;
+        if (more) load(ni, nk0);            // global loads in flight during the MFMAs below
+        if ((wm[i] >> wave) & 1) {
+            const int nks = min(KS, kb - k0);
 #pragma unroll
-        for (int kk = 0; kk < KS; kk += 4) {
-            R ar[2], ai[2], br[2], bi[2];
+            for (int s = 0; s < KS / 4; ++s) {
+                if (4 * s >= nks) break;
+                const int bidx = (wave * 16 + li) * SB + 4 * s + lk;
+                const R br = Br[bidx], bi = Bi[bidx], nbi = -bi;
 #pragma unroll
-            for (int a = 0; a < 2; ++a) {
-                const int m = wm + 16 * a + li, k = kk + lk;
-                const int idx = g.transA ? m * SK + k : k * SA + m;
-                ar[a] = Ar[idx];
-                ai[a] = Ai[idx];
-            }
-#pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                const int idx = (wn + 16 * b + li) * SK + kk + lk;
-                br[b] = Br[idx];
-                bi[b] = Bi[idx];
-            }
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b) {
-                    cr[a][b] = M::op(ar[a], br[b], cr[a][b]);
-                    cr[a][b] = M::op(-ai[a], bi[b], cr[a][b]);
-                    ci[a][b] = M::op(ar[a], bi[b], ci[a][b]);
-                    ci[a][b] = M::op(ai[a], br[b], ci[a][b]);
+                for (int a = 0; a < MBX; ++a) {
+                    if (a >= tl.mb) break;
+                    const int aidx = (4 * s + lk) * SA + 16 * a + li;
+                    const R ar = Ar[aidx], ai = Ai[aidx];
+                    cr[a] = M::op(ar, br, cr[a]);
+                    cr[a] = M::op(ai, nbi, cr[a]);
+                    ci[a] = M::op(ar, bi, ci[a]);
+                    ci[a] = M::op(ai, br, ci[a]);
                 }
+            }
         }
         __syncthreads();
         if (more) {
             store();
             __syncthreads();
         }
+        i = ni;
+        k0 = nk0;
     }
     // D layout of a 16x16 MFMA tile: lane holds rows M::row(lane/16, v), column lane%16
+    const int n = tl.n0 + wave * 16 + li;
+    if (n >= g.ncols) return;
+    V* Cc = C + colC[g.col0 + n];
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
-        const int n = n0 + wn + 16 * b + li;
-        if (n >= g.ncols) continue;
-        V* Cc = C + colC[g.col0 + n];
+    for (int a = 0; a < MBX; ++a) {
+        if (a >= tl.mb) break;
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int m = m0 + wm + 16 * a + M::row(lk, v);
-                if (m >= g.M) continue;
-                R re = cr[a][b][v], im = ci[a][b][v];
-                if (g.soff >= 0) {
-                    const TA s = opm[g.soff + m];
-                    const R sr = (R)s.x, si = (R)s.y;
-                    const R t = re * sr - im * si;
-                    im = re * si + im * sr;
-                    re = t;
-                }
-                if (g.mode == 0) {
-                    Cc[m] = mk<V>(re, im);
-                } else {
-                    V o = Cc[m];
-                    o.x += (typename Real<V>::T)re;
-                    o.y += (typename Real<V>::T)im;
-                    Cc[m] = o;
-                }
+        for (int v = 0; v < 4; ++v) {
+            const int m = tl.m0 + 16 * a + M::row(lk, v);
+            if (m >= g.M) continue;
+            R re = cr[a][v], im = ci[a][v];
+            if (g.soff >= 0) {
+                const TA s = opm[g.soff + m];
+                const R sr = (R)s.x, si = (R)s.y;
+                const R t = re * sr - im * si;
+                im = re * si + im * sr;
+                re = t;
             }
+            if (g.mode == 0) {
+                Cc[m] = mk<V>(re, im);
+            } else {
+                V o = Cc[m];
+                o.x += (T)re;
+                o.y += (T)im;
+                Cc[m] = o;
+            }
+        }
     }
 }
 

@@ -64,10 +64,12 @@ struct GGroup {
     int acol0;      // kb > 0: first A block offset in colA (block b at colA[acol0 + b])
 };
 struct GTile {
-    int g, mt, nt, pad;
+    int g;
+    int m0, n0;     // first row, first column
+    int mb;         // row blocks of 16 (at most GEMM_MB)
 };
-constexpr int GEMM_TM = 64;
-constexpr int GEMM_TN = 64;
+constexpr int GEMM_MB = 7;      // rows per tile <= 112
+constexpr int GEMM_TN = 64;     // columns per tile (16 per wave)
 
 // LF FFT-diagonal M2L (wave3d_eval.cpp:602-661)
 struct FftSrc {

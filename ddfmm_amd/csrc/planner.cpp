@@ -98,8 +98,8 @@ struct Builder {
     int add_group(GGroup g, const std::vector<int64_t>& bc, const std::vector<int64_t>& cc,
                   double* flops, int kb = 0, const std::vector<int64_t>& ablk = {}) {
         if (cc.empty()) return -1;
-        if (kb > 0 && (kb < 32 || (g.K + kb - 1) / kb > 8))
-            throw Error("internal: K blocks must hold at least 32 rows, at most 8 blocks");
+        if (kb > 0 && (g.K % kb != 0 || g.K / kb > 8))
+            throw Error("internal: K must split into at most 8 equal blocks");
         g.kb = kb;
         g.acol0 = (int)pl.colA.size();
         pl.colA.insert(pl.colA.end(), ablk.begin(), ablk.end());
@@ -111,9 +111,14 @@ struct Builder {
         int gi = (int)pl.groups.size();
         pl.groups.push_back(g);
         pl.group_flops.push_back(0);
-        int mt = (g.M + GEMM_TM - 1) / GEMM_TM, nt = (g.ncols + GEMM_TN - 1) / GEMM_TN;
-        for (int a = 0; a < mt; ++a)
-            for (int b = 0; b < nt; ++b) pl.tiles.push_back(GTile{gi, a, b, 0});
+        // rows in blocks of 16 spread evenly over ceil(blocks / GEMM_MB) tiles
+        const int mbt = (g.M + 15) / 16, nmt = (mbt + GEMM_MB - 1) / GEMM_MB;
+        const int nt = (g.ncols + GEMM_TN - 1) / GEMM_TN;
+        for (int a = 0, b0 = 0; a < nmt; ++a) {
+            const int mb = (mbt - b0 + (nmt - a) - 1) / (nmt - a);
+            for (int b = 0; b < nt; ++b) pl.tiles.push_back(GTile{gi, 16 * b0, b * GEMM_TN, mb});
+            b0 += mb;
+        }
         double f = 8.0 * g.M * g.K * g.ncols;
         if (kb > 0) {                        // useful work: present blocks only
             int64_t present = 0;
@@ -177,7 +182,8 @@ struct Builder {
         for (const GTile& t : pl.tiles) {
             need(t.g >= 0 && t.g < (int)pl.groups.size(), "tile group");
             const GGroup& g = pl.groups[t.g];
-            need(t.mt * GEMM_TM < g.M && t.nt * GEMM_TN < g.ncols, "tile index");
+            need(t.mb >= 1 && t.mb <= GEMM_MB && t.m0 >= 0 && t.m0 % 16 == 0 && t.m0 < g.M &&
+                 t.m0 + 16 * (t.mb - 1) < g.M && t.n0 >= 0 && t.n0 < g.ncols, "tile index");
         }
         for (const GGroup& g : pl.groups) {
             need(g.bbuf >= 0 && g.bbuf < BUF_COUNT && g.cbuf >= 0 && g.cbuf < BUF_COUNT, "gemm buffers");
@@ -200,7 +206,7 @@ struct Builder {
                     need(b >= 0 && b + g.K <= pl.bufsize[g.bbuf], "gemm B range");
                 }
             } else {
-                const int nkb = (g.K + g.kb - 1) / g.kb;
+                const int nkb = g.K / g.kb;
                 need(nkb * g.kb == g.K && nkb <= 8, "gemm K blocks");
                 need(g.acol0 + (int64_t)nkb <= (int64_t)pl.colA.size(), "gemm A blocks");
                 for (int h = 0; h < nkb; ++h) need(a_ok(pl.colA[g.acol0 + h], g.kb), "gemm A block");

@@ -30,6 +30,9 @@ def build(verbose: bool = False, jobs: int = 5) -> str:
     cc = hipcc()
     common = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+              # MFMA accumulators in ordinary VGPRs: with the AGPR form the GEMM's
+              # loop-carried accumulators are copied between register files every slab
+              "-mllvm", "-amdgpu-mfma-vgpr-form=1",
               "-Wno-unused-result", "-I", CSRC]
     procs, objs = [], []
     for s in SOURCES:

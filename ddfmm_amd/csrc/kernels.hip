@@ -214,31 +214,31 @@ template <> struct Mfma<double> {
 constexpr int MAXKB = 8;           // at most 8 K blocks (children of a box)
 
 // LDS layouts (conflict-free fragment reads for wave64 MFMA 16x16x4):
-//   A slab [k][m], row stride SA = 16 * GEMM_MB = 112 words = 16 mod 32: the two k rows of
-//   a 32-lane f32 read land in disjoint bank halves (f64 reads serve 16 lanes per cycle);
+//   A slab [k][m], row stride SA = 16 MB words, or 16 MB + 16 for even MB, so that
+//   SA = 16 mod 32: the two k rows of a 32-lane f32 read land in disjoint bank halves
+//   (f64 reads serve 16 lanes per cycle, one k row);
 //   B slab [n][k], row stride SB: f32 18 (even/odd rows split the banks), f64 17 (odd).
 template <class R> struct Lds;
 template <> struct Lds<float> { static constexpr int SB = 18; };
 template <> struct Lds<double> { static constexpr int SB = 17; };
 
 // C[:, ccol] (=|+=) diag(scale) * op(A) * B[:, bcol] on MFMA 16x16x4 (f32 or f64).
-// A tile covers up to GEMM_MB row blocks of 16 (the group's rows split evenly over its
-// tiles) and GEMM_TN = 64 columns, 16 per wave; complex products as four real MFMAs.
-// K runs over segments (the whole K, or the K blocks present in at least one of the
-// tile's columns) in slabs of 16 with a register-prefetch pipeline: slab s+1 is loaded
-// while slab s runs on the matrix cores.  Padding stays within 16 rows, 4 k and 16
-// columns: the MFMAs stop at the last k step of a segment, and a wave whose columns are
-// all empty, or all lack the current K block, skips the segment's MFMAs.
-template <class TA, class V, class R>
+// A tile covers MB row blocks of 16 (the group's rows split evenly over its tiles; tiles
+// are launched per MB) and GEMM_TN = 64 columns, 16 per wave; complex products as four
+// real MFMAs.  K runs over segments (the whole K, or the K blocks present in at least
+// one of the tile's columns) in slabs of 16 with a register-prefetch pipeline: slab s+1
+// is loaded while slab s runs on the matrix cores.  Padding stays within 16 rows, 4 k
+// and 16 columns: the MFMAs stop at the last k step of a segment, and a wave whose
+// columns are all empty, or all lack the current K block, skips the segment's MFMAs.
+template <class TA, class V, class R, int MB>
 __global__ __launch_bounds__(256) void gemm_kernel(
     const GGroup* __restrict__ groups, const GTile* __restrict__ tiles,
     const int64_t* __restrict__ colB, const int64_t* __restrict__ colC, const int64_t* __restrict__ colA,
     const TA* __restrict__ opm, BufTable bt, int tile0) {
     using M = Mfma<R>;
     using T = typename Real<V>::T;
-    constexpr int KS = 16, MBX = GEMM_MB, TM = 16 * MBX, SA = TM, SB = Lds<R>::SB;
-    constexpr int NA = TM * KS / 256, NB = GEMM_TN * KS / 256;
-    static_assert(NA * 256 == TM * KS && NB * 256 == GEMM_TN * KS, "slab must split evenly over 256 threads");
+    constexpr int KS = 16, TM = 16 * MB, SA = (MB & 1) ? TM : TM + 16, SB = Lds<R>::SB;
+    constexpr int NA = TM * KS / 256, NB = GEMM_TN * KS / 256;   // elements per thread and slab
     const GTile tl = tiles[tile0 + xcd_item(blockIdx.x, gridDim.x)];
     const GGroup g = groups[tl.g];
     const int rhs = blockIdx.y;
@@ -252,7 +252,6 @@ __global__ __launch_bounds__(256) void gemm_kernel(
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
-    const int mrows = min(16 * tl.mb, g.M - tl.m0);     // rows of this tile holding data
     const int kb = g.kb ? g.kb : g.K;                   // segment length
     const int nkb = g.kb ? g.K / g.kb : 1;
     if (tid < MAXKB * 4) used[tid >> 2][tid & 3] = 0;
@@ -282,8 +281,15 @@ __global__ __launch_bounds__(256) void gemm_kernel(
     if (tid < nact) ao[tid] = g.kb ? colA[g.acol0 + act[tid]] : g.aoff;
     __syncthreads();
 
-    // element e of a slab: A (row, k) and B (k, column)
-    auto a_at = [&](int e, int& mm, int& kk) {
+    // per-thread slab elements, fixed for the tile: A (row, k) and B (k, column)
+    const int mrows = g.M - tl.m0;                      // rows >= mrows are zero
+    const int64_t kstride = g.transA ? 1 : g.lda;       // A offset per k
+    int64_t aoffs[NA];
+    int akk[NA], alds[NA];
+#pragma unroll
+    for (int l = 0; l < NA; ++l) {
+        const int e = tid + 256 * l;
+        int mm, kk;
         if (g.transA) {              // global A[k + m*lda]: 4 consecutive k per row
             kk = (e & 3) | ((e / (4 * TM)) << 2);
             mm = (e >> 2) % TM;
@@ -291,58 +297,61 @@ __global__ __launch_bounds__(256) void gemm_kernel(
             mm = e % TM;
             kk = e / TM;
         }
-    };
+        const int64_t m = tl.m0 + mm;
+        aoffs[l] = g.transA ? kk + m * g.lda : m + kk * (int64_t)g.lda;
+        akk[l] = mm < mrows ? kk : KS;                  // KS: never loaded
+        alds[l] = kk * SA + mm;
+    }
+    int64_t bc[NB];                                     // column offsets of the current segment
+    int bkk[NB], blds[NB];
+#pragma unroll
+    for (int l = 0; l < NB; ++l) {
+        const int e = tid + 256 * l;
+        bkk[l] = e % KS;
+        blds[l] = (e / KS) * SB + (e % KS);
+    }
     TA ra[NA];
     V rb[NB];
-    auto load = [&](int i, int k0) {
-        const TA* Ab = opm + ao[i];
-        const int hb = g.kb ? act[i] : 0;
+    auto load = [&](int i, int k0, bool newseg) {
+        const TA* Ab = opm + ao[i] + k0 * kstride;
         const int kl = kb - k0;
+        if (newseg) {
+            const int hb = g.kb ? act[i] : 0;
+#pragma unroll
+            for (int l = 0; l < NB; ++l) bc[l] = bo[hb][(tid + 256 * l) / KS];
+        }
 #pragma unroll
         for (int l = 0; l < NA; ++l) {
-            int mm, kk;
-            a_at(tid + 256 * l, mm, kk);
             TA a;
             a.x = 0;
             a.y = 0;
-            if (mm < mrows && kk < kl) {
-                const int m = tl.m0 + mm, k = k0 + kk;
-                a = g.transA ? Ab[k + (int64_t)m * g.lda] : Ab[m + (int64_t)k * g.lda];
-            }
+            if (akk[l] < kl) a = Ab[aoffs[l]];
             ra[l] = a;
         }
+        const V* Bk = B + k0;
 #pragma unroll
         for (int l = 0; l < NB; ++l) {
-            const int e = tid + 256 * l;
-            const int kk = e % KS, nn = e / KS;
             V b = mk<V>(0, 0);
-            const int64_t c = bo[hb][nn];
-            if (c >= 0 && kk < kl) b = B[c + k0 + kk];
+            if (bc[l] >= 0 && bkk[l] < kl) b = Bk[bc[l] + bkk[l]];
             rb[l] = b;
         }
     };
     auto store = [&]() {
 #pragma unroll
         for (int l = 0; l < NA; ++l) {
-            int mm, kk;
-            a_at(tid + 256 * l, mm, kk);
-            if (mm < 16 * tl.mb) {
-                Ar[kk * SA + mm] = (R)ra[l].x;
-                Ai[kk * SA + mm] = (R)ra[l].y;
-            }
+            Ar[alds[l]] = (R)ra[l].x;
+            Ai[alds[l]] = (R)ra[l].y;
         }
 #pragma unroll
         for (int l = 0; l < NB; ++l) {
-            const int e = tid + 256 * l;
-            const int idx = (e / KS) * SB + (e % KS);
-            Br[idx] = (R)rb[l].x;
-            Bi[idx] = (R)rb[l].y;
+            Br[blds[l]] = (R)rb[l].x;
+            Bi[blds[l]] = (R)rb[l].y;
         }
     };
 
-    typename M::acc4 cr[MBX], ci[MBX];
+    typename M::acc4 cr[MB], ci[MB];
 #pragma unroll
-    for (int a = 0; a < MBX; ++a)
+    for (int a = 0; a < MB; ++a)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             cr[a][v] = 0;
@@ -350,10 +359,11 @@ __global__ __launch_bounds__(256) void gemm_kernel(
         }
 
     if (nact > 0) {
-        load(0, 0);
+        load(0, 0, true);
         store();
     }
     __syncthreads();
+    const int bb = (wave * 16 + li) * SB + lk, aa = lk * SA + li;
     int i = 0, k0 = 0;
     while (i < nact) {
         int ni = i, nk0 = k0 + KS;
@@ -362,19 +372,14 @@ __global__ __launch_bounds__(256) void gemm_kernel(
             nk0 = 0;
         }
         const bool more = ni < nact;
-        if (more) load(ni, nk0);            // global loads in flight during the MFMAs below
+        if (more) load(ni, nk0, ni != i);   // global loads in flight during the MFMAs below
         if ((wm[i] >> wave) & 1) {
-            const int nks = min(KS, kb - k0);
+            const int nsteps = (min(KS, kb - k0) + 3) >> 2;
+            for (int s = 0; s < nsteps; ++s) {
+                const R br = Br[bb + 4 * s], bi = Bi[bb + 4 * s], nbi = -bi;
 #pragma unroll
-            for (int s = 0; s < KS / 4; ++s) {
-                if (4 * s >= nks) break;
-                const int bidx = (wave * 16 + li) * SB + 4 * s + lk;
-                const R br = Br[bidx], bi = Bi[bidx], nbi = -bi;
-#pragma unroll
-                for (int a = 0; a < MBX; ++a) {
-                    if (a >= tl.mb) break;
-                    const int aidx = (4 * s + lk) * SA + 16 * a + li;
-                    const R ar = Ar[aidx], ai = Ai[aidx];
+                for (int a = 0; a < MB; ++a) {
+                    const R ar = Ar[aa + 4 * s * SA + 16 * a], ai = Ai[aa + 4 * s * SA + 16 * a];
                     cr[a] = M::op(ar, br, cr[a]);
                     cr[a] = M::op(ai, nbi, cr[a]);
                     ci[a] = M::op(ar, bi, ci[a]);
@@ -395,8 +400,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(
     if (n >= g.ncols) return;
     V* Cc = C + colC[g.col0 + n];
 #pragma unroll
-    for (int a = 0; a < MBX; ++a) {
-        if (a >= tl.mb) break;
+    for (int a = 0; a < MB; ++a) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int m = tl.m0 + 16 * a + M::row(lk, v);
@@ -419,6 +423,21 @@ __global__ __launch_bounds__(256) void gemm_kernel(
             }
         }
     }
+}
+
+template <class TA, class V, class R>
+void launch_gemm(int mb, int n, int nrhs, hipStream_t st, const DevPlan& d, const TA* opm, int t0) {
+    const dim3 grid(n, nrhs);
+#define DDFMM_GEMM_MB(X)                                                                                \
+    case X:                                                                                             \
+        gemm_kernel<TA, V, R, X><<<grid, 256, 0, st>>>(d.groups, d.tiles, d.colB, d.colC, d.colA, opm, \
+                                                       d.bt, t0);                                       \
+        break;
+    switch (mb) {
+        DDFMM_GEMM_MB(1) DDFMM_GEMM_MB(2) DDFMM_GEMM_MB(3) DDFMM_GEMM_MB(4) DDFMM_GEMM_MB(5)
+        DDFMM_GEMM_MB(6) DDFMM_GEMM_MB(7)
+    }
+#undef DDFMM_GEMM_MB
 }
 
 // ---------------------------------------------------------------- (2P)^3 DFT in LDS
@@ -671,16 +690,16 @@ hipError_t launch_stage_t(const DevPlan& d, const Stage& s, int nrhs, hipStream_
         }
         break;
     case ST_GEMM:
-        if (d.mixed) {
-            if (s.prec == PREC_SOLVE)
-                gemm_kernel<double2, float2, double><<<dim3(n, nrhs), 256, 0, st>>>(
-                    d.groups, d.tiles, d.colB, d.colC, d.colA, d.opmat, d.bt, s.begin);
+        // one launch per tile height (the stage's tiles are sorted by row blocks)
+        for (int mb = 1, t0 = s.begin; mb <= GEMM_MB; t0 = s.mbend[mb], ++mb) {
+            const int cnt = s.mbend[mb] - t0;
+            if (cnt <= 0) continue;
+            if (d.mixed && s.prec == PREC_SOLVE)
+                launch_gemm<double2, float2, double>(mb, cnt, nrhs, st, d, d.opmat, t0);
+            else if (d.mixed)
+                launch_gemm<float2, float2, float>(mb, cnt, nrhs, st, d, d.opmat32, t0);
             else
-                gemm_kernel<float2, float2, float><<<dim3(n, nrhs), 256, 0, st>>>(
-                    d.groups, d.tiles, d.colB, d.colC, d.colA, d.opmat32, d.bt, s.begin);
-        } else {
-            gemm_kernel<double2, double2, double><<<dim3(n, nrhs), 256, 0, st>>>(
-                d.groups, d.tiles, d.colB, d.colC, d.colA, d.opmat, d.bt, s.begin);
+                launch_gemm<double2, double2, double>(mb, cnt, nrhs, st, d, d.opmat, t0);
         }
         break;
     case ST_FFTFWD:

@@ -16,6 +16,7 @@
 #include "planner.hpp"
 
 #include <algorithm>
+#include <set>
 #include <cstring>
 #include <sstream>
 #include <string>
@@ -136,6 +137,17 @@ struct Builder {
         std::memset(&s, 0, sizeof(s));
         s.kind = kind; s.begin = b; s.end = e; s.buf = buf; s.count = count;
         std::strncpy(s.name, name, sizeof(s.name) - 1);
+        if (kind == ST_GEMM) {
+            // tiles sorted by height (one kernel instance per height), groups kept together
+            std::stable_sort(pl.tiles.begin() + b, pl.tiles.begin() + e,
+                             [](const GTile& x, const GTile& y) { return x.mb < y.mb; });
+            s.mbend[0] = b;
+            for (int mb = 1, t = b; mb <= GEMM_MB; ++mb) {
+                while (t < e && pl.tiles[t].mb == mb) ++t;
+                s.mbend[mb] = t;
+            }
+            if (s.mbend[GEMM_MB] != e) throw Error("internal: tile height out of range");
+        }
         // solves: LF pinv factors and HF E1/E2/E3 chains
         const std::string nm(name);
         s.prec = (nm.find("uc2ue") != std::string::npos || nm.find("dc2de") != std::string::npos ||
@@ -1013,13 +1025,9 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
     for (auto& s : pl.stages) {
         double w = 0;
         if (s.kind == ST_GEMM) {
-            int last = -1;
-            for (int t = s.begin; t < s.end; ++t) {
-                int gi = pl.tiles[t].g;
-                if (gi == last) continue;
-                last = gi;
-                w += pl.group_flops[gi];
-            }
+            std::set<int> seen;
+            for (int t = s.begin; t < s.end; ++t)
+                if (seen.insert(pl.tiles[t].g).second) w += pl.group_flops[pl.tiles[t].g];
         } else if (s.kind == ST_KAPPLY) {
             for (int t = s.begin; t < s.end; ++t) {
                 const KTask& k = pl.ktasks[t];

@@ -690,17 +690,13 @@ hipError_t launch_stage_t(const DevPlan& d, const Stage& s, int nrhs, hipStream_
         }
         break;
     case ST_GEMM:
-        // one launch per tile height (the stage's tiles are sorted by row blocks)
-        for (int mb = 1, t0 = s.begin; mb <= GEMM_MB; t0 = s.mbend[mb], ++mb) {
-            const int cnt = s.mbend[mb] - t0;
-            if (cnt <= 0) continue;
-            if (d.mixed && s.prec == PREC_SOLVE)
-                launch_gemm<double2, float2, double>(mb, cnt, nrhs, st, d, d.opmat, t0);
-            else if (d.mixed)
-                launch_gemm<float2, float2, float>(mb, cnt, nrhs, st, d, d.opmat32, t0);
-            else
-                launch_gemm<double2, double2, double>(mb, cnt, nrhs, st, d, d.opmat, t0);
-        }
+        if (s.mb < 1 || s.mb > GEMM_MB) return hipErrorInvalidValue;
+        if (d.mixed && s.prec == PREC_SOLVE)
+            launch_gemm<double2, float2, double>(s.mb, n, nrhs, st, d, d.opmat, s.begin);
+        else if (d.mixed)
+            launch_gemm<float2, float2, float>(s.mb, n, nrhs, st, d, d.opmat32, s.begin);
+        else
+            launch_gemm<double2, double2, double>(s.mb, n, nrhs, st, d, d.opmat, s.begin);
         break;
     case ST_FFTFWD:
         if (d.P != 6) return hipErrorInvalidValue;

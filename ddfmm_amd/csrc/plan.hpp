@@ -126,7 +126,8 @@ struct Stage {
     int pad;
     double work;        // per RHS: GEMM useful flops, KAPPLY interactions, FFT spectra, V-list pairs,
                         // reduction elements read
-    int mbend[GEMM_MB + 1];   // ST_GEMM: tiles of height mb are [mbend[mb-1], mbend[mb]), mbend[0] = begin
+    int mb;             // ST_GEMM: row blocks of every tile in the range (one kernel instance)
+    int pad2;
     char name[24];
 };
 

@@ -137,22 +137,27 @@ struct Builder {
         std::memset(&s, 0, sizeof(s));
         s.kind = kind; s.begin = b; s.end = e; s.buf = buf; s.count = count;
         std::strncpy(s.name, name, sizeof(s.name) - 1);
-        if (kind == ST_GEMM) {
-            // tiles sorted by height (one kernel instance per height), groups kept together
-            std::stable_sort(pl.tiles.begin() + b, pl.tiles.begin() + e,
-                             [](const GTile& x, const GTile& y) { return x.mb < y.mb; });
-            s.mbend[0] = b;
-            for (int mb = 1, t = b; mb <= GEMM_MB; ++mb) {
-                while (t < e && pl.tiles[t].mb == mb) ++t;
-                s.mbend[mb] = t;
-            }
-            if (s.mbend[GEMM_MB] != e) throw Error("internal: tile height out of range");
-        }
         // solves: LF pinv factors and HF E1/E2/E3 chains
         const std::string nm(name);
         s.prec = (nm.find("uc2ue") != std::string::npos || nm.find("dc2de") != std::string::npos ||
                   nm.find("hf_up_e") != std::string::npos || nm.find("hf_dn_e") != std::string::npos)
                      ? PREC_SOLVE : PREC_TRANSLATION;
+        if (kind == ST_GEMM) {
+            // one launch per tile height (a kernel instance each): tiles sorted by height,
+            // the tiles of a group kept together
+            std::stable_sort(pl.tiles.begin() + b, pl.tiles.begin() + e,
+                             [](const GTile& x, const GTile& y) { return x.mb < y.mb; });
+            for (int t = b; t < e;) {
+                int u = t;
+                while (u < e && pl.tiles[u].mb == pl.tiles[t].mb) ++u;
+                if (pl.tiles[t].mb < 1 || pl.tiles[t].mb > GEMM_MB) throw Error("internal: tile height out of range");
+                Stage q = s;
+                q.begin = t; q.end = u; q.mb = pl.tiles[t].mb;
+                pl.stages.push_back(q);
+                t = u;
+            }
+            return;
+        }
         pl.stages.push_back(s);
     }
     int64_t gen(int64_t toff, int m, const double ts[3], int tneg, int64_t soff, int n,
@@ -184,7 +189,10 @@ struct Builder {
         const int64_t G3 = (int64_t)(2 * pl.P) * (2 * pl.P) * (2 * pl.P);
         for (const Stage& st : pl.stages) {
             need(st.begin >= 0 && st.begin <= st.end, "stage range");
-            if (st.kind == ST_GEMM) need(st.end <= (int64_t)pl.tiles.size(), "gemm stage");
+            if (st.kind == ST_GEMM) {
+                need(st.end <= (int64_t)pl.tiles.size(), "gemm stage");
+                for (int t = st.begin; t < st.end; ++t) need(pl.tiles[t].mb == st.mb, "gemm stage tile height");
+            }
             if (st.kind == ST_KAPPLY) need(st.end <= (int64_t)pl.ktasks.size(), "kapply stage");
             if (st.kind == ST_FFTFWD) need(st.end <= (int64_t)pl.fsrc.size(), "fft stage");
             if (st.kind == ST_VLIST) need(st.end <= (int64_t)pl.vtrg.size(), "vlist stage");
@@ -1025,9 +1033,22 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
     for (auto& s : pl.stages) {
         double w = 0;
         if (s.kind == ST_GEMM) {
-            std::set<int> seen;
-            for (int t = s.begin; t < s.end; ++t)
-                if (seen.insert(pl.tiles[t].g).second) w += pl.group_flops[pl.tiles[t].g];
+            // useful flops of each tile: its rows x (K, or the present K blocks) x columns
+            for (int t = s.begin; t < s.end; ++t) {
+                const GTile& tl = pl.tiles[t];
+                const GGroup& g = pl.groups[tl.g];
+                const double rows = std::min(16 * tl.mb, g.M - tl.m0);
+                const int n1 = std::min(tl.n0 + GEMM_TN, g.ncols);
+                double k = 0;
+                if (g.kb == 0) {
+                    k = (double)g.K * (n1 - tl.n0);
+                } else {
+                    const int nkb = g.K / g.kb;
+                    for (int n = tl.n0; n < n1; ++n)
+                        for (int h = 0; h < nkb; ++h) k += pl.colB[g.bcol0 + (int64_t)n * nkb + h] >= 0 ? g.kb : 0;
+                }
+                w += 8.0 * rows * k;
+            }
         } else if (s.kind == ST_KAPPLY) {
             for (int t = s.begin; t < s.end; ++t) {
                 const KTask& k = pl.ktasks[t];

@@ -79,3 +79,48 @@ def test_eval_without_device_is_rejected(oplib):
             _lib.check(_lib.lib().ddfmm_eval(h, den.ctypes.data, den.ctypes.data, 1))
     finally:
         _lib.lib().ddfmm_destroy(h)
+
+
+def _host_stages(oplib, pos, K, precision):
+    prm = _lib.Params(K=K, ACCU=2, NPQ=4, ptsmax=100, maxlevel=12, numC=1, kernel=0, nrhs_max=1,
+                      precision=precision)
+    h = ctypes.c_void_p()
+    _lib.check(_lib.lib().ddfmm_create(-1, ctypes.byref(h)))
+    try:
+        _lib.check(_lib.lib().ddfmm_setup(h, pos.ctypes.data, len(pos), ctypes.byref(prm), oplib[0].encode(),
+                                          oplib[1].encode()))
+        cnt = ctypes.c_int()
+        _lib.check(_lib.lib().ddfmm_stage_times(h, 0, None, None, None, None, ctypes.byref(cnt)))
+        n = cnt.value
+        names = ctypes.create_string_buffer(24 * n)
+        work = (ctypes.c_double * n)()
+        kinds = (ctypes.c_int * n)()
+        _lib.check(_lib.lib().ddfmm_stage_times(h, n, names, None, work, kinds, ctypes.byref(cnt)))
+        st = _lib.Stats()
+        _lib.check(_lib.lib().ddfmm_get_stats(h, ctypes.byref(st)))
+        return ([(names.raw[24 * i:24 * i + 24].split(b"\0")[0].decode(), work[i], kinds[i] % 16, kinds[i] // 16)
+                 for i in range(n)], st.as_dict())
+    finally:
+        _lib.lib().ddfmm_destroy(h)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_stage_precision_classes_and_work(oplib, precision):
+    """Solve stages (check -> equivalent: LF pinv, HF E1 E2 E3 chains) carry the solve class
+    that the mixed mode runs in FP64; every GEMM launch (one per tile height) keeps it; the
+    per-launch useful flops add up to the planner's GEMM total."""
+    pos = geometry.sphere(20000, 8.0, 0)
+    stages, st = _host_stages(oplib, pos, 8.0, precision)
+    ST_GEMM, ST_REDUCE = 4, 7
+    solve_names = ("uc2ue", "dc2de", "hf_up_e", "hf_dn_e")
+    gemm = [s for s in stages if s[2] == ST_GEMM]
+    assert gemm and any(s[2] == ST_REDUCE for s in stages)
+    for name, _, kind, prec in stages:
+        assert prec == (1 if any(t in name for t in solve_names) else 0), name
+    names = {s[0] for s in gemm}
+    if precision == 1:
+        assert {"lf_uc2ue", "lf_dc2de", "hf_up_e", "hf_dn_et"} <= names
+    else:
+        assert {"lf_uc2ue_1", "lf_uc2ue_2", "hf_up_e3", "hf_dn_e1t"} <= names
+    total = sum(s[1] for s in gemm)
+    assert abs(total - st["gemm_flops"]) <= 1e-9 * st["gemm_flops"]

@@ -374,17 +374,40 @@ __global__ __launch_bounds__(256) void gemm_kernel(
         const bool more = ni < nact;
         if (more) load(ni, nk0, ni != i);   // global loads in flight during the MFMAs below
         if ((wm[i] >> wave) & 1) {
+            // fragments of k step s+1 are read from LDS while step s runs on the matrix
+            // cores; each step issues four sweeps over the row blocks, so consecutive
+            // MFMAs never accumulate into the same registers
             const int nsteps = (min(KS, kb - k0) + 3) >> 2;
-            for (int s = 0; s < nsteps; ++s) {
-                const R br = Br[bb + 4 * s], bi = Bi[bb + 4 * s], nbi = -bi;
+            R ar[MB], ai[MB], br, bi;
+            auto frag = [&](int s, R* xr, R* xi, R& yr, R& yi) {
+                yr = Br[bb + 4 * s];
+                yi = Bi[bb + 4 * s];
 #pragma unroll
                 for (int a = 0; a < MB; ++a) {
-                    const R ar = Ar[aa + 4 * s * SA + 16 * a], ai = Ai[aa + 4 * s * SA + 16 * a];
-                    cr[a] = M::op(ar, br, cr[a]);
-                    cr[a] = M::op(ai, nbi, cr[a]);
-                    ci[a] = M::op(ar, bi, ci[a]);
-                    ci[a] = M::op(ai, br, ci[a]);
+                    xr[a] = Ar[aa + 4 * s * SA + 16 * a];
+                    xi[a] = Ai[aa + 4 * s * SA + 16 * a];
                 }
+            };
+            frag(0, ar, ai, br, bi);
+            for (int s = 0; s < nsteps; ++s) {
+                R nar[MB], nai[MB], nbr, nbi_;
+                frag(min(s + 1, KS / 4 - 1), nar, nai, nbr, nbi_);
+                const R mbi = -bi;
+#pragma unroll
+                for (int a = 0; a < MB; ++a) cr[a] = M::op(ar[a], br, cr[a]);
+#pragma unroll
+                for (int a = 0; a < MB; ++a) ci[a] = M::op(ar[a], bi, ci[a]);
+#pragma unroll
+                for (int a = 0; a < MB; ++a) cr[a] = M::op(ai[a], mbi, cr[a]);
+#pragma unroll
+                for (int a = 0; a < MB; ++a) ci[a] = M::op(ai[a], br, ci[a]);
+#pragma unroll
+                for (int a = 0; a < MB; ++a) {
+                    ar[a] = nar[a];
+                    ai[a] = nai[a];
+                }
+                br = nbr;
+                bi = nbi_;
             }
         }
         __syncthreads();
@@ -440,101 +463,137 @@ void launch_gemm(int mb, int n, int nrhs, hipStream_t st, const DevPlan& d, cons
 #undef DDFMM_GEMM_MB
 }
 
-// ---------------------------------------------------------------- (2P)^3 DFT in LDS
-template <int G, class V>
-struct Dft {
-    // twiddles w_k = exp(-2 pi i k / G) (forward); the inverse uses their conjugates
-    __device__ static void twiddles(V* tw) {
-        if ((int)threadIdx.x < G) {
-            double s, c;
-            sincospi(2.0 * threadIdx.x / G, &s, &c);
-            tw[threadIdx.x] = mk<V>(c, -s);
-        }
-    }
-    // one pass of a length-G DFT along axis ax (0 fastest) from in to out
-    __device__ static void pass(const V* in, V* out, const V* tw, int ax, bool inverse) {
-        constexpr int G3 = G * G * G;
-        const int st = ax == 0 ? 1 : (ax == 1 ? G : G * G);
-        for (int e = threadIdx.x; e < G3; e += blockDim.x) {
-            const int p = (e / st) % G;
-            const int base = e - p * st;
-            V acc = mk<V>(0, 0);
-#pragma unroll
-            for (int q = 0; q < G; ++q) {
-                V w = tw[(p * q) % G];
-                if (inverse) w.y = -w.y;
-                cfma(acc, in[base + q * st], w);
-            }
-            out[e] = acc;
-        }
-    }
+// ---------------------------------------------------------------- pruned (2P)^3 DFT, P = 6
+// The equivalent surface of a box sits on the 6^3 sub-cube [6, 12)^3 of the 12^3 grid
+// (grid index = round((x + W/2)/step) + P), so the forward transform has non-zero input
+// only there and the inverse transform is sampled only there: each 1-D pass runs on the
+// lines that carry data and produces the outputs later passes use (18144 complex MACs
+// per 3-D transform instead of 62208).  One thread per line, twiddles as constants.
+struct Tw12 {
+    static constexpr double c[12] = {1.0, 0.86602540378443865, 0.5, 0.0, -0.5, -0.86602540378443865,
+                                     -1.0, -0.86602540378443865, -0.5, 0.0, 0.5, 0.86602540378443865};
+    static constexpr double s[12] = {0.0, 0.5, 0.86602540378443865, 1.0, 0.86602540378443865, 0.5,
+                                     0.0, -0.5, -0.86602540378443865, -1.0, -0.86602540378443865, -0.5};
 };
 
-template <int G, class V>
-__global__ __launch_bounds__(256) void fft_fwd_kernel(const FftSrc* __restrict__ fs, const int* __restrict__ grid,
-                                                      int NL, BufTable bt, int f0) {
-    constexpr int G3 = G * G * G;
-    __shared__ V g0[G3], g1[G3];
-    __shared__ V tw[G];
-    Dft<G, V>::twiddles(tw);
-    const FftSrc f = fs[f0 + xcd_item(blockIdx.x, gridDim.x)];
-    const int rhs = blockIdx.y;
-    const V* in = bufp<V>(bt, BUF_LF_UPEQ) + (int64_t)rhs * bt.stride[BUF_LF_UPEQ] + f.in_off;
-    V* out = bufp<V>(bt, BUF_SPEC) + (int64_t)rhs * bt.stride[BUF_SPEC] + f.spec_off;
-    for (int e = threadIdx.x; e < G3; e += blockDim.x) g0[e] = mk<V>(0, 0);
-    __syncthreads();
-    for (int k = threadIdx.x; k < NL; k += blockDim.x) g0[grid[f.grid + k]] = in[k];
-    __syncthreads();
-    Dft<G, V>::pass(g0, g1, tw, 0, false);
-    __syncthreads();
-    Dft<G, V>::pass(g1, g0, tw, 1, false);
-    __syncthreads();
-    Dft<G, V>::pass(g0, g1, tw, 2, false);
-    __syncthreads();
-    for (int e = threadIdx.x; e < G3; e += blockDim.x) out[e] = g1[e];
+// out[o * os] = sum_{q < NI} in[q * is] * exp(SGN * 2 pi i (q + QOFF)(o + OOFF) / 12), o < NO
+template <class V, int NI, int NO, int QOFF, int OOFF, int SGN>
+__device__ __forceinline__ void dft12_line(const V* in, int is, V* out, int os) {
+    using T = typename Real<V>::T;
+    V x[NI];
+#pragma unroll
+    for (int q = 0; q < NI; ++q) x[q] = in[q * is];
+#pragma unroll
+    for (int o = 0; o < NO; ++o) {
+        V acc = mk<V>(0, 0);
+#pragma unroll
+        for (int q = 0; q < NI; ++q) {
+            const int m = ((q + QOFF) * (o + OOFF)) % 12;
+            const V w = mk<V>((T)Tw12::c[m], (T)(SGN * Tw12::s[m]));
+            cfma(acc, x[q], w);
+        }
+        out[o * os] = acc;
+    }
 }
 
-template <int G, class V>
+template <class V>
+__global__ __launch_bounds__(256) void fft_fwd_kernel(const FftSrc* __restrict__ fs, const int* __restrict__ grid,
+                                                      int NL, BufTable bt, int f0) {
+    __shared__ V sub[216], A[432], B[864];
+    const FftSrc f = fs[f0 + xcd_item(blockIdx.x, gridDim.x)];
+    const int rhs = blockIdx.y;
+    const int t = threadIdx.x;
+    const V* in = bufp<V>(bt, BUF_LF_UPEQ) + (int64_t)rhs * bt.stride[BUF_LF_UPEQ] + f.in_off;
+    V* out = bufp<V>(bt, BUF_SPEC) + (int64_t)rhs * bt.stride[BUF_SPEC] + f.spec_off;
+    if (t < 216) sub[t] = mk<V>(0, 0);
+    __syncthreads();
+    for (int k = t; k < NL; k += blockDim.x) {
+        const int g = grid[f.grid + k];
+        sub[(g % 12 - 6) + 6 * ((g / 12) % 12 - 6) + 36 * (g / 144 - 6)] = in[k];
+    }
+    __syncthreads();
+    if (t < 36)          // x lines (y', z'): 6 inputs -> 12 outputs, A[z'][y'][kx]
+        dft12_line<V, 6, 12, 6, 0, -1>(sub + 6 * t, 1, A + 12 * t, 1);
+    __syncthreads();
+    if (t < 72)          // y lines (kx, z'): A -> B[z'][ky][kx]
+        dft12_line<V, 6, 12, 6, 0, -1>(A + (t % 12) + 72 * (t / 12), 12, B + (t % 12) + 144 * (t / 12), 12);
+    __syncthreads();
+    if (t < 144)         // z lines (kx, ky): B -> spectrum [kz][ky][kx]
+        dft12_line<V, 6, 12, 6, 0, -1>(B + t, 144, out + t, 144);
+}
+
+template <class V>
 __global__ __launch_bounds__(256) void vlist_kernel(const VTarget* __restrict__ vt, const VPair* __restrict__ vp,
                                                     const int* __restrict__ grid, const V* __restrict__ opm,
                                                     int NL, BufTable bt, int v0) {
-    constexpr int G3 = G * G * G;
+    constexpr int G3 = 1728;
     constexpr int PER = (G3 + 255) / 256;
-    __shared__ V g0[G3], g1[G3];
-    __shared__ V tw[G];
-    Dft<G, V>::twiddles(tw);
-    const VTarget t = vt[v0 + xcd_item(blockIdx.x, gridDim.x)];
+    __shared__ V g0[G3], A[864], B[432];
+    const VTarget tg = vt[v0 + xcd_item(blockIdx.x, gridDim.x)];
     const int rhs = blockIdx.y;
+    const int t = threadIdx.x;
     const V* spec = bufp<V>(bt, BUF_SPEC) + (int64_t)rhs * bt.stride[BUF_SPEC];
     V acc[PER];
 #pragma unroll
     for (int l = 0; l < PER; ++l) acc[l] = mk<V>(0, 0);
-    for (int p = t.p0; p < t.p1; ++p) {
-        const VPair pr = vp[p];
-        const V* s = spec + pr.spec_off;
-        const V* T = opm + t.tab_off + (int64_t)pr.offidx * G3;
+    // Hadamard products, two source pairs per step (both pairs' loads in flight together)
+    int p = tg.p0;
+    for (; p + 1 < tg.p1; p += 2) {
+        const VPair pa = vp[p], pb = vp[p + 1];
+        const V* sa = spec + pa.spec_off;
+        const V* sb = spec + pb.spec_off;
+        const V* Ta = opm + tg.tab_off + (int64_t)pa.offidx * G3;
+        const V* Tb = opm + tg.tab_off + (int64_t)pb.offidx * G3;
+        V xa[PER], ya[PER], xb[PER], yb[PER];
 #pragma unroll
         for (int l = 0; l < PER; ++l) {
-            const int e = threadIdx.x + 256 * l;
-            if (e < G3) cfma(acc[l], s[e], T[e]);
+            const int e = t + 256 * l;
+            if (e < G3) {
+                xa[l] = sa[e];
+                ya[l] = Ta[e];
+                xb[l] = sb[e];
+                yb[l] = Tb[e];
+            }
+        }
+#pragma unroll
+        for (int l = 0; l < PER; ++l) {
+            const int e = t + 256 * l;
+            if (e < G3) {
+                cfma(acc[l], xa[l], ya[l]);
+                cfma(acc[l], xb[l], yb[l]);
+            }
+        }
+    }
+    if (p < tg.p1) {
+        const VPair pa = vp[p];
+        const V* sa = spec + pa.spec_off;
+        const V* Ta = opm + tg.tab_off + (int64_t)pa.offidx * G3;
+#pragma unroll
+        for (int l = 0; l < PER; ++l) {
+            const int e = t + 256 * l;
+            if (e < G3) cfma(acc[l], sa[e], Ta[e]);
         }
     }
 #pragma unroll
     for (int l = 0; l < PER; ++l) {
-        const int e = threadIdx.x + 256 * l;
+        const int e = t + 256 * l;
         if (e < G3) g0[e] = acc[l];
     }
     __syncthreads();
-    Dft<G, V>::pass(g0, g1, tw, 0, true);
+    if (t < 144)         // x lines (ky, kz): 12 inputs -> x' in [6, 12), A[kz][ky][x']
+        dft12_line<V, 12, 6, 0, 6, 1>(g0 + 12 * t, 1, A + 6 * t, 1);
     __syncthreads();
-    Dft<G, V>::pass(g1, g0, tw, 1, true);
+    if (t < 72)          // y lines (x', kz): A -> B[kz][y'][x']
+        dft12_line<V, 12, 6, 0, 6, 1>(A + (t % 6) + 72 * (t / 6), 6, B + (t % 6) + 36 * (t / 6), 6);
     __syncthreads();
-    Dft<G, V>::pass(g0, g1, tw, 2, true);
+    if (t < 36)          // z lines (x', y'): B -> C[z'][y'][x'] (reuses A)
+        dft12_line<V, 12, 6, 0, 6, 1>(B + t, 36, A + t, 36);
     __syncthreads();
-    V* out = bufp<V>(bt, BUF_LF_CHK) + (int64_t)rhs * bt.stride[BUF_LF_CHK] + t.out_off;
+    V* out = bufp<V>(bt, BUF_LF_CHK) + (int64_t)rhs * bt.stride[BUF_LF_CHK] + tg.out_off;
     const typename Real<V>::T coef = (typename Real<V>::T)(1.0 / G3);
-    for (int k = threadIdx.x; k < NL; k += blockDim.x) {
-        const V v = g1[grid[t.grid + k]];
+    for (int k = t; k < NL; k += blockDim.x) {
+        const int g = grid[tg.grid + k];
+        const V v = A[(g % 12 - 6) + 6 * ((g / 12) % 12 - 6) + 36 * (g / 144 - 6)];
         V o = out[k];
         o.x += v.x * coef;
         o.y += v.y * coef;
@@ -700,15 +759,15 @@ hipError_t launch_stage_t(const DevPlan& d, const Stage& s, int nrhs, hipStream_
         break;
     case ST_FFTFWD:
         if (d.P != 6) return hipErrorInvalidValue;
-        fft_fwd_kernel<12, V><<<dim3(n, nrhs), 256, 0, st>>>(d.fsrc, d.grid, d.NL, d.bt, s.begin);
+        fft_fwd_kernel<V><<<dim3(n, nrhs), 256, 0, st>>>(d.fsrc, d.grid, d.NL, d.bt, s.begin);
         break;
     case ST_VLIST:
         if (d.P != 6) return hipErrorInvalidValue;
         if (d.mixed)
-            vlist_kernel<12, float2><<<dim3(n, nrhs), 256, 0, st>>>(d.vtrg, d.vpair, d.grid, d.opmat32, d.NL, d.bt,
+            vlist_kernel<float2><<<dim3(n, nrhs), 256, 0, st>>>(d.vtrg, d.vpair, d.grid, d.opmat32, d.NL, d.bt,
                                                                     s.begin);
         else
-            vlist_kernel<12, double2><<<dim3(n, nrhs), 256, 0, st>>>(d.vtrg, d.vpair, d.grid, d.opmat, d.NL, d.bt,
+            vlist_kernel<double2><<<dim3(n, nrhs), 256, 0, st>>>(d.vtrg, d.vpair, d.grid, d.opmat, d.NL, d.bt,
                                                                      s.begin);
         break;
     case ST_REDUCE:

@@ -24,14 +24,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "hbm": 8.0e12}
-ST_ZERO, ST_GATHER, ST_SCATTER, ST_KAPPLY, ST_GEMM, ST_FFTFWD, ST_VLIST, ST_REDUCE = range(8)
+ST_ZERO, ST_GATHER, ST_SCATTER, ST_KAPPLY, ST_GEMM, ST_FFTFWD, ST_VLIST, ST_REDUCE, ST_FORK, ST_JOIN = range(10)
 
 
 def kernel_class(kind: int, prec: int) -> str:
     if kind == ST_GEMM:
         return "gemm_solve" if prec == 1 else "gemm_translation"
     return {ST_KAPPLY: "kapply", ST_VLIST: "vlist", ST_FFTFWD: "fft_fwd", ST_ZERO: "zero",
-            ST_GATHER: "gather", ST_SCATTER: "scatter", ST_REDUCE: "reduce"}[kind]
+            ST_GATHER: "gather", ST_SCATTER: "scatter", ST_REDUCE: "reduce", ST_FORK: "sync", ST_JOIN: "sync"}[kind]
 
 
 def log(msg):
@@ -184,6 +184,8 @@ def main():
     for _ in range(reps):
         w.eval_device(dd, vd, nrhs)
         for name, ms, work, kind, prec in w.stage_times():
+            if kind in (ST_FORK, ST_JOIN):
+                continue
             ps = per_stage.setdefault(name, [0.0, 0.0])
             ps[0] += ms / reps
             ps[1] += work * nrhs / reps

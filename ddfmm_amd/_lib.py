@@ -11,7 +11,8 @@ LIB_PATH = os.path.join(HERE, "libddfmm.so")
 
 EXPORTS = ["ddfmm_last_error", "ddfmm_create", "ddfmm_destroy", "ddfmm_setup", "ddfmm_eval",
            "ddfmm_eval_device", "ddfmm_direct_sum", "ddfmm_kernel_matrix", "ddfmm_get_stats",
-           "ddfmm_dump_lists", "ddfmm_read_buffer", "ddfmm_set_timing", "ddfmm_stage_times"]
+           "ddfmm_dump_lists", "ddfmm_read_buffer", "ddfmm_set_timing", "ddfmm_stage_times",
+           "ddfmm_stage_exec"]
 
 
 class Params(ctypes.Structure):
@@ -64,6 +65,7 @@ def lib():
     L.ddfmm_set_timing.argtypes = [vp, ci]
     L.ddfmm_stage_times.argtypes = [vp, ci, ctypes.c_char_p, ctypes.POINTER(cd), ctypes.POINTER(cd),
                                      ctypes.POINTER(ci), ctypes.POINTER(ci)]
+    L.ddfmm_stage_exec.argtypes = [vp, ci, ctypes.POINTER(cd), ctypes.POINTER(ci)]
     for name in EXPORTS:
         if name != "ddfmm_last_error":
             getattr(L, name).restype = ci

@@ -38,6 +38,8 @@ struct HipFail : std::runtime_error {
 struct ddfmm_ctx {
     int device = -1;
     hipStream_t stream = nullptr;
+    hipStream_t aux = nullptr;                 // second stream of an evaluation (near field, LF lists)
+    hipEvent_t sync_ev[STAGE_EVENTS] = {};     // fork/join events between the two streams
     Params prm;
     int nrhs_max = 1, precision = 0;
     HostPlan hp;
@@ -86,6 +88,8 @@ int ddfmm_create(int device, ddfmm_ctx** out) {
         try {
             HIPCHK(hipSetDevice(device));
             HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+            HIPCHK(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
+            for (auto& e : c->sync_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         } catch (const std::exception& e) {
             delete c;
             return fail(DDFMM_EDEVICE, e.what());
@@ -100,7 +104,11 @@ int ddfmm_destroy(ddfmm_ctx* c) {
     if (c->device >= 0) {
         (void)hipSetDevice(c->device);
         if (c->stream) (void)hipStreamSynchronize(c->stream);
+        if (c->aux) (void)hipStreamSynchronize(c->aux);
         c->release();
+        for (auto e : c->sync_ev)
+            if (e) (void)hipEventDestroy(e);
+        if (c->aux) (void)hipStreamDestroy(c->aux);
         if (c->stream) (void)hipStreamDestroy(c->stream);
     }
     delete c;
@@ -204,28 +212,42 @@ int ddfmm_setup(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_params* 
     return DDFMM_OK;
 }
 
+// Stages run on the evaluation stream `st` or on the context's second stream; ST_FORK
+// makes the second stream wait for the work issued so far on `st`, ST_JOIN the reverse,
+// so that the caller's stream sees a complete evaluation.  With timing on, each stage is
+// bracketed by events on its own stream.
 static int run_stages(ddfmm_ctx* c, int nrhs, hipStream_t st) {
     const auto& stages = c->hp.stages;
-    if (c->timing) {
-        if (c->ev.size() != stages.size() + 1) {
-            for (auto e : c->ev) (void)hipEventDestroy(e);
-            c->ev.assign(stages.size() + 1, nullptr);
-            for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
-        }
-        HIPCHK(hipEventRecord(c->ev[0], st));
+    if (c->timing && c->ev.size() != 2 * stages.size()) {
+        for (auto e : c->ev) (void)hipEventDestroy(e);
+        c->ev.assign(2 * stages.size(), nullptr);
+        for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
     }
     for (size_t i = 0; i < stages.size(); ++i) {
-        hipError_t e = launch_stage(c->dp, stages[i], nrhs, st);
-        if (e != hipSuccess)
-            throw HipFail(std::string("stage ") + stages[i].name + ": " + hipGetErrorString(e));
-        if (c->timing) HIPCHK(hipEventRecord(c->ev[i + 1], st));
+        const Stage& s = stages[i];
+        if (s.kind == ST_FORK) {
+            HIPCHK(hipEventRecord(c->sync_ev[s.buf], st));
+            HIPCHK(hipStreamWaitEvent(c->aux, c->sync_ev[s.buf], 0));
+            continue;
+        }
+        if (s.kind == ST_JOIN) {
+            HIPCHK(hipEventRecord(c->sync_ev[s.buf], c->aux));
+            HIPCHK(hipStreamWaitEvent(st, c->sync_ev[s.buf], 0));
+            continue;
+        }
+        hipStream_t ss = s.stream ? c->aux : st;
+        if (c->timing) HIPCHK(hipEventRecord(c->ev[2 * i], ss));
+        hipError_t e = launch_stage(c->dp, s, nrhs, ss);
+        if (e != hipSuccess) throw HipFail(std::string("stage ") + s.name + ": " + hipGetErrorString(e));
+        if (c->timing) HIPCHK(hipEventRecord(c->ev[2 * i + 1], ss));
     }
     if (c->timing) {
-        HIPCHK(hipEventSynchronize(c->ev.back()));
+        HIPCHK(hipStreamSynchronize(st));
         c->stage_ms.assign(stages.size(), 0.0);
         for (size_t i = 0; i < stages.size(); ++i) {
+            if (stages[i].kind == ST_FORK || stages[i].kind == ST_JOIN) continue;
             float ms = 0;
-            HIPCHK(hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
+            HIPCHK(hipEventElapsedTime(&ms, c->ev[2 * i], c->ev[2 * i + 1]));
             c->stage_ms[i] = ms;
         }
     }
@@ -385,6 +407,15 @@ int ddfmm_stage_times(ddfmm_ctx* c, int cap, char* names, double* ms, double* wo
         if (work) work[i] = st[i].work;
         if (kinds) kinds[i] = st[i].kind + 16 * st[i].prec;
     }
+    return DDFMM_OK;
+}
+
+int ddfmm_stage_exec(ddfmm_ctx* c, int cap, double* exec, int* count) {
+    if (!c || !count) return fail(DDFMM_EINVAL, "null argument");
+    const auto& st = c->hp.stages;
+    *count = (int)st.size();
+    for (int i = 0; i < (int)st.size() && i < cap; ++i)
+        if (exec) exec[i] = st[i].exec;
     return DDFMM_OK;
 }
 

@@ -107,13 +107,16 @@ template <class V, int RC>
 __global__ __launch_bounds__(KA_THREADS) void kapply_kernel(
     const KTask* __restrict__ tasks, const KSrc* __restrict__ srcs,
     const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
-    BufTable bt, int task0, int nrhs, int ktype) {
-    const KTask t = tasks[task0 + xcd_item(blockIdx.x, gridDim.x)];
+    BufTable bt, int task0, int ntask, int nrhs, int ktype) {
     const int r0 = blockIdx.y * RC;
     const int nr = min(RC, nrhs - r0);
     __shared__ double sx[KA_THREADS], sy[KA_THREADS], sz[KA_THREADS];
     __shared__ V sv[RC][KA_THREADS];
     __shared__ V red[RC][KA_THREADS];
+    // grid-stride over the tasks (a grid smaller than the task count keeps a launch on the
+    // second stream to a few blocks per CU); gridDim.x is a multiple of 8 or >= ntask
+    for (int bi = blockIdx.x; bi < ntask; bi += gridDim.x) {
+    const KTask t = tasks[task0 + xcd_item(bi, ntask)];
     for (int tb = 0; tb < t.count; tb += KA_THREADS) {
         const int nt = min(KA_THREADS, t.count - tb);
         const int G = KA_THREADS / nt;
@@ -189,6 +192,7 @@ __global__ __launch_bounds__(KA_THREADS) void kapply_kernel(
         }
         __syncthreads();
     }
+    }
 }
 
 // ---------------------------------------------------------------- grouped complex GEMM (MFMA)
@@ -244,8 +248,11 @@ __global__ __launch_bounds__(256) void gemm_kernel(
     const int rhs = blockIdx.y;
     const V* B = bufp<V>(bt, g.bbuf) + (int64_t)rhs * bt.stride[g.bbuf];
     V* C = bufp<V>(bt, g.cbuf) + (int64_t)rhs * bt.stride[g.cbuf];
-    __shared__ R Ar[KS * SA], Ai[KS * SA];
-    __shared__ R Br[GEMM_TN * SB], Bi[GEMM_TN * SB];
+    // f32: two LDS slab buffers (one barrier per slab); f64 slabs are twice the size and
+    // take twice the MFMA time per k, so one buffer and two barriers per slab
+    constexpr int NBUF = sizeof(R) == 4 ? 2 : 1;
+    __shared__ R Ar[NBUF][KS * SA], Ai[NBUF][KS * SA];
+    __shared__ R Br[NBUF][GEMM_TN * SB], Bi[NBUF][GEMM_TN * SB];
     __shared__ int64_t bo[MAXKB][GEMM_TN];   // B offset of (block, column), -1 = absent
     __shared__ int64_t ao[MAXKB];            // A offset of each active segment
     __shared__ int used[MAXKB][4], act[MAXKB], wm[MAXKB], nact_s;
@@ -254,32 +261,6 @@ __global__ __launch_bounds__(256) void gemm_kernel(
     const int li = lane & 15, lk = lane >> 4;
     const int kb = g.kb ? g.kb : g.K;                   // segment length
     const int nkb = g.kb ? g.K / g.kb : 1;
-    if (tid < MAXKB * 4) used[tid >> 2][tid & 3] = 0;
-    __syncthreads();
-    for (int e = tid; e < nkb * GEMM_TN; e += 256) {
-        const int h = e / GEMM_TN, j = e % GEMM_TN;
-        int64_t v = -1;
-        if (tl.n0 + j < g.ncols) v = g.kb ? colB[g.bcol0 + (int64_t)(tl.n0 + j) * nkb + h] : colB[g.bcol0 + tl.n0 + j];
-        bo[h][j] = v;
-        if (v >= 0) used[h][j >> 4] = 1;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        int c = 0;
-        for (int h = 0; h < nkb; ++h) {
-            const int w = used[h][0] | (used[h][1] << 1) | (used[h][2] << 2) | (used[h][3] << 3);
-            if (w) {
-                act[c] = h;
-                wm[c] = w;
-                ++c;
-            }
-        }
-        nact_s = c;
-    }
-    __syncthreads();
-    const int nact = nact_s;
-    if (tid < nact) ao[tid] = g.kb ? colA[g.acol0 + act[tid]] : g.aoff;
-    __syncthreads();
 
     // per-thread slab elements, fixed for the tile: A (row, k) and B (k, column)
     const int mrows = g.M - tl.m0;                      // rows >= mrows are zero
@@ -310,15 +291,57 @@ __global__ __launch_bounds__(256) void gemm_kernel(
         bkk[l] = e % KS;
         blds[l] = (e / KS) * SB + (e % KS);
     }
+    // this lane's output column, fetched early (the epilogue needs it last)
+    const int ncol = tl.n0 + wave * 16 + li;
+    const int64_t coff = ncol < g.ncols ? colC[g.col0 + ncol] : -1;
+
+    // segments.  Contiguous B (kb = 0): one segment, column offsets read straight into
+    // registers, waves with columns run.  Blocked B: a table of the K blocks present in
+    // the tile's columns, with per-wave masks.
+    int nact = 1, wm0 = 0;
+    if (g.kb == 0) {
+        const int nv = g.ncols - tl.n0;
+        wm0 = nv >= GEMM_TN ? 15 : (1 << ((nv + 15) >> 4)) - 1;
+#pragma unroll
+        for (int l = 0; l < NB; ++l) {
+            const int n = tl.n0 + (tid + 256 * l) / KS;
+            bc[l] = n < g.ncols ? colB[g.bcol0 + n] : -1;
+        }
+    } else {
+        if (tid < MAXKB * 4) used[tid >> 2][tid & 3] = 0;
+        __syncthreads();
+        for (int e = tid; e < nkb * GEMM_TN; e += 256) {
+            const int h = e / GEMM_TN, j = e % GEMM_TN;
+            const int64_t v = tl.n0 + j < g.ncols ? colB[g.bcol0 + (int64_t)(tl.n0 + j) * nkb + h] : -1;
+            bo[h][j] = v;
+            if (v >= 0) used[h][j >> 4] = 1;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int c = 0;
+            for (int h = 0; h < nkb; ++h) {
+                const int w = used[h][0] | (used[h][1] << 1) | (used[h][2] << 2) | (used[h][3] << 3);
+                if (w) {
+                    act[c] = h;
+                    wm[c] = w;
+                    ++c;
+                }
+            }
+            nact_s = c;
+        }
+        __syncthreads();
+        nact = nact_s;
+        if (tid < nact) ao[tid] = colA[g.acol0 + act[tid]];
+        __syncthreads();
+    }
     TA ra[NA];
     V rb[NB];
     auto load = [&](int i, int k0, bool newseg) {
-        const TA* Ab = opm + ao[i] + k0 * kstride;
+        const TA* Ab = opm + (g.kb ? ao[i] : g.aoff) + k0 * kstride;
         const int kl = kb - k0;
         if (newseg) {
-            const int hb = g.kb ? act[i] : 0;
 #pragma unroll
-            for (int l = 0; l < NB; ++l) bc[l] = bo[hb][(tid + 256 * l) / KS];
+            for (int l = 0; l < NB; ++l) bc[l] = bo[act[i]][(tid + 256 * l) / KS];
         }
 #pragma unroll
         for (int l = 0; l < NA; ++l) {
@@ -336,16 +359,16 @@ __global__ __launch_bounds__(256) void gemm_kernel(
             rb[l] = b;
         }
     };
-    auto store = [&]() {
+    auto store = [&](int q) {
 #pragma unroll
         for (int l = 0; l < NA; ++l) {
-            Ar[alds[l]] = (R)ra[l].x;
-            Ai[alds[l]] = (R)ra[l].y;
+            Ar[q][alds[l]] = (R)ra[l].x;
+            Ai[q][alds[l]] = (R)ra[l].y;
         }
 #pragma unroll
         for (int l = 0; l < NB; ++l) {
-            Br[blds[l]] = (R)rb[l].x;
-            Bi[blds[l]] = (R)rb[l].y;
+            Br[q][blds[l]] = (R)rb[l].x;
+            Bi[q][blds[l]] = (R)rb[l].y;
         }
     };
 
@@ -358,70 +381,90 @@ __global__ __launch_bounds__(256) void gemm_kernel(
             ci[a][v] = 0;
         }
 
-    if (nact > 0) {
-        load(0, 0, true);
-        store();
-    }
-    __syncthreads();
     const int bb = (wave * 16 + li) * SB + lk, aa = lk * SA + li;
-    int i = 0, k0 = 0;
-    while (i < nact) {
-        int ni = i, nk0 = k0 + KS;
+    // MFMAs of slab (i, k0) from LDS buffer q: four sweeps over the row blocks per k step,
+    // so consecutive MFMAs never accumulate into the same registers
+    auto mfma_slab = [&](int i, int k0, int q) {
+        if (!((((g.kb ? wm[i] : wm0)) >> wave) & 1)) return;
+        const int nsteps = (min(KS, kb - k0) + 3) >> 2;
+        for (int s = 0; s < nsteps; ++s) {
+            const R br = Br[q][bb + 4 * s], bi = Bi[q][bb + 4 * s], mbi = -bi;
+            R ar[MB], ai[MB];
+#pragma unroll
+            for (int a = 0; a < MB; ++a) {
+                ar[a] = Ar[q][aa + 4 * s * SA + 16 * a];
+                ai[a] = Ai[q][aa + 4 * s * SA + 16 * a];
+            }
+#pragma unroll
+            for (int a = 0; a < MB; ++a) cr[a] = M::op(ar[a], br, cr[a]);
+#pragma unroll
+            for (int a = 0; a < MB; ++a) ci[a] = M::op(ar[a], bi, ci[a]);
+#pragma unroll
+            for (int a = 0; a < MB; ++a) cr[a] = M::op(ai[a], mbi, cr[a]);
+#pragma unroll
+            for (int a = 0; a < MB; ++a) ci[a] = M::op(ai[a], br, ci[a]);
+        }
+    };
+    auto next = [&](int i, int k0, int& ni, int& nk0) {
+        ni = i;
+        nk0 = k0 + KS;
         if (nk0 >= kb) {
             ni = i + 1;
             nk0 = 0;
         }
-        const bool more = ni < nact;
-        if (more) load(ni, nk0, ni != i);   // global loads in flight during the MFMAs below
-        if ((wm[i] >> wave) & 1) {
-            // fragments of k step s+1 are read from LDS while step s runs on the matrix
-            // cores; each step issues four sweeps over the row blocks, so consecutive
-            // MFMAs never accumulate into the same registers
-            const int nsteps = (min(KS, kb - k0) + 3) >> 2;
-            R ar[MB], ai[MB], br, bi;
-            auto frag = [&](int s, R* xr, R* xi, R& yr, R& yi) {
-                yr = Br[bb + 4 * s];
-                yi = Bi[bb + 4 * s];
-#pragma unroll
-                for (int a = 0; a < MB; ++a) {
-                    xr[a] = Ar[aa + 4 * s * SA + 16 * a];
-                    xi[a] = Ai[aa + 4 * s * SA + 16 * a];
-                }
-            };
-            frag(0, ar, ai, br, bi);
-            for (int s = 0; s < nsteps; ++s) {
-                R nar[MB], nai[MB], nbr, nbi_;
-                frag(min(s + 1, KS / 4 - 1), nar, nai, nbr, nbi_);
-                const R mbi = -bi;
-#pragma unroll
-                for (int a = 0; a < MB; ++a) cr[a] = M::op(ar[a], br, cr[a]);
-#pragma unroll
-                for (int a = 0; a < MB; ++a) ci[a] = M::op(ar[a], bi, ci[a]);
-#pragma unroll
-                for (int a = 0; a < MB; ++a) cr[a] = M::op(ai[a], mbi, cr[a]);
-#pragma unroll
-                for (int a = 0; a < MB; ++a) ci[a] = M::op(ai[a], br, ci[a]);
-#pragma unroll
-                for (int a = 0; a < MB; ++a) {
-                    ar[a] = nar[a];
-                    ai[a] = nai[a];
-                }
-                br = nbr;
-                bi = nbi_;
-            }
-        }
+    };
+    if (nact > 0) {
+        load(0, 0, g.kb != 0);
+        store(0);
+    }
+    int i = 0, k0 = 0;
+    if constexpr (NBUF == 2) {
+        // slab s+1 waits in registers while slab s runs; it is written to the other buffer
+        // after the MFMAs, then slab s+2 is fetched; one barrier per slab
+        int ni, nk0;
+        next(i, k0, ni, nk0);
+        bool more = ni < nact;
+        if (more) load(ni, nk0, ni != i);
         __syncthreads();
-        if (more) {
-            store();
+        int q = 0;
+        while (i < nact) {
+            mfma_slab(i, k0, q);
+            int n2i = nact, n2k0 = 0;
+            bool more2 = false;
+            if (more) {
+                store(q ^ 1);
+                next(ni, nk0, n2i, n2k0);
+                more2 = n2i < nact;
+                if (more2) load(n2i, n2k0, n2i != ni);
+            }
             __syncthreads();
+            i = ni;
+            k0 = nk0;
+            ni = n2i;
+            nk0 = n2k0;
+            more = more2;
+            q ^= 1;
         }
-        i = ni;
-        k0 = nk0;
+    } else {
+        __syncthreads();
+        while (i < nact) {
+            int ni, nk0;
+            next(i, k0, ni, nk0);
+            const bool more = ni < nact;
+            if (more) load(ni, nk0, ni != i);   // global loads in flight during the MFMAs below
+            mfma_slab(i, k0, 0);
+            __syncthreads();
+            if (more) {
+                store(0);
+                __syncthreads();
+            }
+            i = ni;
+            k0 = nk0;
+        }
     }
     // D layout of a 16x16 MFMA tile: lane holds rows M::row(lane/16, v), column lane%16
-    const int n = tl.n0 + wave * 16 + li;
-    if (n >= g.ncols) return;
-    V* Cc = C + colC[g.col0 + n];
+    if (coff < 0) return;
+    V* Cc = C + coff;
 #pragma unroll
     for (int a = 0; a < MB; ++a) {
 #pragma unroll
@@ -498,11 +541,13 @@ __device__ __forceinline__ void dft12_line(const V* in, int is, V* out, int os) 
 
 template <class V>
 __global__ __launch_bounds__(256) void fft_fwd_kernel(const FftSrc* __restrict__ fs, const int* __restrict__ grid,
-                                                      int NL, BufTable bt, int f0) {
+                                                      int NL, BufTable bt, int f0, int nf) {
     __shared__ V sub[216], A[432], B[864];
-    const FftSrc f = fs[f0 + xcd_item(blockIdx.x, gridDim.x)];
     const int rhs = blockIdx.y;
     const int t = threadIdx.x;
+    for (int bi = blockIdx.x; bi < nf; bi += gridDim.x) {
+    const FftSrc f = fs[f0 + xcd_item(bi, nf)];
+    __syncthreads();
     const V* in = bufp<V>(bt, BUF_LF_UPEQ) + (int64_t)rhs * bt.stride[BUF_LF_UPEQ] + f.in_off;
     V* out = bufp<V>(bt, BUF_SPEC) + (int64_t)rhs * bt.stride[BUF_SPEC] + f.spec_off;
     if (t < 216) sub[t] = mk<V>(0, 0);
@@ -520,19 +565,22 @@ __global__ __launch_bounds__(256) void fft_fwd_kernel(const FftSrc* __restrict__
     __syncthreads();
     if (t < 144)         // z lines (kx, ky): B -> spectrum [kz][ky][kx]
         dft12_line<V, 6, 12, 6, 0, -1>(B + t, 144, out + t, 144);
+    }
 }
 
 template <class V>
 __global__ __launch_bounds__(256) void vlist_kernel(const VTarget* __restrict__ vt, const VPair* __restrict__ vp,
                                                     const int* __restrict__ grid, const V* __restrict__ opm,
-                                                    int NL, BufTable bt, int v0) {
+                                                    int NL, BufTable bt, int v0, int nv) {
     constexpr int G3 = 1728;
     constexpr int PER = (G3 + 255) / 256;
     __shared__ V g0[G3], A[864], B[432];
-    const VTarget tg = vt[v0 + xcd_item(blockIdx.x, gridDim.x)];
     const int rhs = blockIdx.y;
     const int t = threadIdx.x;
     const V* spec = bufp<V>(bt, BUF_SPEC) + (int64_t)rhs * bt.stride[BUF_SPEC];
+    for (int bi = blockIdx.x; bi < nv; bi += gridDim.x) {
+    const VTarget tg = vt[v0 + xcd_item(bi, nv)];
+    __syncthreads();
     V acc[PER];
 #pragma unroll
     for (int l = 0; l < PER; ++l) acc[l] = mk<V>(0, 0);
@@ -598,6 +646,7 @@ __global__ __launch_bounds__(256) void vlist_kernel(const VTarget* __restrict__ 
         o.x += v.x * coef;
         o.y += v.y * coef;
         out[k] = o;
+    }
     }
 }
 
@@ -718,6 +767,11 @@ __global__ __launch_bounds__(256) void reduce_kernel(const RTarget* __restrict__
     }
 }
 
+// Grid caps of the launches on the second stream (multiples of 8: one XCD round), for
+// running them as a background beside the GEMMs; very large = one block per item
+constexpr int AUX_KAPPLY_BLOCKS = 1 << 30;
+constexpr int AUX_BLOCKS = 1 << 30;
+
 int grid_for(int64_t n, int threads) {
     int64_t b = (n + threads - 1) / threads;
     return (int)std::min<int64_t>(std::max<int64_t>(b, 1), 65536);
@@ -739,15 +793,17 @@ hipError_t launch_stage_t(const DevPlan& d, const Stage& s, int nrhs, hipStream_
         scatter_kernel<V><<<grid_for(d.N * nrhs, 256), 256, 0, st>>>((const V*)d.bt.p[BUF_VAL], d.val_io, d.perm, d.N,
                                                                      nrhs);
         break;
-    case ST_KAPPLY:
+    case ST_KAPPLY: {
+        const int gx = s.stream ? std::min(n, AUX_KAPPLY_BLOCKS) : n;
         if (nrhs == 1) {
-            kapply_kernel<V, 1><<<dim3(n, 1), KA_THREADS, 0, st>>>(d.ktasks, d.ksrcs, d.px, d.py, d.pz, d.bt,
-                                                                   s.begin, nrhs, d.ktype);
+            kapply_kernel<V, 1><<<dim3(gx, 1), KA_THREADS, 0, st>>>(d.ktasks, d.ksrcs, d.px, d.py, d.pz, d.bt,
+                                                                    s.begin, n, nrhs, d.ktype);
         } else {
-            kapply_kernel<V, 4><<<dim3(n, (nrhs + 3) / 4), KA_THREADS, 0, st>>>(d.ktasks, d.ksrcs, d.px, d.py, d.pz,
-                                                                               d.bt, s.begin, nrhs, d.ktype);
+            kapply_kernel<V, 4><<<dim3(gx, (nrhs + 3) / 4), KA_THREADS, 0, st>>>(d.ktasks, d.ksrcs, d.px, d.py, d.pz,
+                                                                                d.bt, s.begin, n, nrhs, d.ktype);
         }
         break;
+    }
     case ST_GEMM:
         if (s.mb < 1 || s.mb > GEMM_MB) return hipErrorInvalidValue;
         if (d.mixed && s.prec == PREC_SOLVE)
@@ -759,16 +815,17 @@ hipError_t launch_stage_t(const DevPlan& d, const Stage& s, int nrhs, hipStream_
         break;
     case ST_FFTFWD:
         if (d.P != 6) return hipErrorInvalidValue;
-        fft_fwd_kernel<V><<<dim3(n, nrhs), 256, 0, st>>>(d.fsrc, d.grid, d.NL, d.bt, s.begin);
+        fft_fwd_kernel<V><<<dim3(s.stream ? std::min(n, AUX_BLOCKS) : n, nrhs), 256, 0, st>>>(d.fsrc, d.grid, d.NL,
+                                                                                              d.bt, s.begin, n);
         break;
     case ST_VLIST:
         if (d.P != 6) return hipErrorInvalidValue;
         if (d.mixed)
-            vlist_kernel<float2><<<dim3(n, nrhs), 256, 0, st>>>(d.vtrg, d.vpair, d.grid, d.opmat32, d.NL, d.bt,
-                                                                    s.begin);
+            vlist_kernel<float2><<<dim3(s.stream ? std::min(n, AUX_BLOCKS) : n, nrhs), 256, 0, st>>>(
+                d.vtrg, d.vpair, d.grid, d.opmat32, d.NL, d.bt, s.begin, n);
         else
-            vlist_kernel<double2><<<dim3(n, nrhs), 256, 0, st>>>(d.vtrg, d.vpair, d.grid, d.opmat, d.NL, d.bt,
-                                                                     s.begin);
+            vlist_kernel<double2><<<dim3(s.stream ? std::min(n, AUX_BLOCKS) : n, nrhs), 256, 0, st>>>(
+                d.vtrg, d.vpair, d.grid, d.opmat, d.NL, d.bt, s.begin, n);
         break;
     case ST_REDUCE:
         reduce_kernel<V><<<dim3((n + 3) / 4, nrhs), 256, 0, st>>>(d.rtrg, n, d.bt, s.begin);

@@ -110,8 +110,14 @@ struct KGen {
 };
 
 enum StageKind : int {
-    ST_ZERO = 0, ST_GATHER, ST_SCATTER, ST_KAPPLY, ST_GEMM, ST_FFTFWD, ST_VLIST, ST_REDUCE
+    ST_ZERO = 0, ST_GATHER, ST_SCATTER, ST_KAPPLY, ST_GEMM, ST_FFTFWD, ST_VLIST, ST_REDUCE,
+    ST_FORK,        // the second stream waits for everything issued so far on the first (event buf)
+    ST_JOIN         // the first stream waits for everything issued so far on the second (event buf)
 };
+// Two streams per evaluation: the caller's (0) carries the HF pipeline and the LF passes
+// that depend on it; the second (1) runs, concurrently, the work that needs only the
+// densities (near field) or only the LF upward pass (FFT, V list, X list).
+constexpr int STAGE_EVENTS = 3;   // fork after gather, fork after the LF upward pass, join
 // Arithmetic class of a stage.  In the mixed-precision mode translation stages
 // (kernel-generated matrices: M2M/L2L, U2U/D2D child kernels, HF M2L, near field, V list)
 // run in FP32 on FP32 vectors, while the ill-conditioned check->equivalent solves
@@ -120,14 +126,15 @@ enum StagePrec : int { PREC_TRANSLATION = 0, PREC_SOLVE = 1 };
 struct Stage {
     int kind;
     int begin, end;     // descriptor range (tasks / tiles / sources / targets)
-    int buf;            // ST_ZERO: buffer id
+    int buf;            // ST_ZERO: buffer id; ST_FORK/ST_JOIN: event index
     int64_t count;      // ST_ZERO: elements per rhs
     int prec;           // StagePrec
-    int pad;
+    int stream;         // 0: the evaluation (caller's) stream, 1: the concurrent second stream
     double work;        // per RHS: GEMM useful flops, KAPPLY interactions, FFT spectra, V-list pairs,
                         // reduction elements read
     int mb;             // ST_GEMM: row blocks of every tile in the range (one kernel instance)
     int pad2;
+    double exec;        // ST_GEMM: MFMA flops per RHS the kernel executes (padding included)
     char name[24];
 };
 

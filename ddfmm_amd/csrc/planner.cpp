@@ -58,6 +58,10 @@ CMat matmul(const CMat& A, const CMat& B) {
     return C;
 }
 
+// which independent sections run on the second stream
+constexpr bool SECOND_STREAM_NEAR = true;   // near field (U list, direct W/X lists)
+constexpr bool SECOND_STREAM_LF = true;     // FFT, V list, X list per LF width
+
 struct Builder {
     const Params& prm;
     const Mlib& mlib;
@@ -131,7 +135,49 @@ struct Builder {
         if (flops) *flops += f;
         return gi;
     }
+    // Stages are emitted in dependency order per section and merged at the end:
+    //   0 gather | fork 0 | 1 near field (stream 1) | 2 LF upward | fork 1 |
+    //   3 FFT, V list, X list per LF width (stream 1) | 4 HF upward | 5 HF downward |
+    //   join | 6 W=1 -> LF reduction, LF downward, leaf far field, scatter
+    int cur_sec = 0;
+    std::vector<int> stage_sec;
     void stage(int kind, int b, int e, const char* name, int buf = -1, int64_t count = 0) {
+        const size_t n0 = pl.stages.size();
+        stage_emit(kind, b, e, name, buf, count);
+        stage_sec.resize(pl.stages.size(), cur_sec);
+        (void)n0;
+    }
+    void merge_sections() {
+        std::vector<Stage> out;
+        auto sync = [&](int kind, int ev, const char* nm) {
+            Stage q;
+            std::memset(&q, 0, sizeof(q));
+            q.kind = kind; q.buf = ev;
+            std::strncpy(q.name, nm, sizeof(q.name) - 1);
+            out.push_back(q);
+        };
+        auto emit = [&](int sec) {
+            for (size_t i = 0; i < pl.stages.size(); ++i)
+                if (stage_sec[i] == sec) {
+                    Stage q = pl.stages[i];
+                    q.stream = ((sec == 1 && SECOND_STREAM_NEAR) || (sec == 3 && SECOND_STREAM_LF)) ? 1 : 0;
+                    out.push_back(q);
+                }
+        };
+        emit(0);
+        sync(ST_FORK, 0, "fork");
+        emit(1);
+        emit(2);
+        sync(ST_FORK, 1, "fork");
+        emit(3);
+        emit(4);
+        emit(5);
+        sync(ST_JOIN, 2, "join");
+        emit(6);
+        pl.stages.swap(out);
+        stage_sec.clear();
+    }
+    void stage_emit(int kind, int b, int e, const char* name, int buf, int64_t count) {
         if (kind != ST_ZERO && kind != ST_GATHER && kind != ST_SCATTER && e <= b) return;
         Stage s;
         std::memset(&s, 0, sizeof(s));
@@ -198,6 +244,8 @@ struct Builder {
             if (st.kind == ST_VLIST) need(st.end <= (int64_t)pl.vtrg.size(), "vlist stage");
             if (st.kind == ST_REDUCE) need(st.end <= (int64_t)pl.rtrg.size(), "reduce stage");
             if (st.kind == ST_ZERO) need(st.buf >= 0 && st.buf < BUF_COUNT && st.count <= pl.bufsize[st.buf], "zero");
+            if (st.kind == ST_FORK || st.kind == ST_JOIN) need(st.buf >= 0 && st.buf < STAGE_EVENTS, "sync event");
+            need(st.stream == 0 || st.stream == 1, "stage stream");
         }
         for (const GTile& t : pl.tiles) {
             need(t.g >= 0 && t.g < (int)pl.groups.size(), "tile group");
@@ -645,7 +693,9 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
     pl.bufsize[BUF_RED] = red_size;
 
     // ================= stages
+    cur_sec = 0;
     stage(ST_GATHER, 0, 0, "gather");
+    cur_sec = 2;
 
     // ---- LF upward (W ascending = finest first), EvalUpwardLow 418-490
     for (auto& kv : ldmap) {
@@ -720,6 +770,7 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
         }
     }
     stage(ST_ZERO, 0, 0, "zero_lfchk", BUF_LF_CHK, pl.bufsize[BUF_LF_CHK]);
+    cur_sec = 4;
 
     // ---- HF upward (W = 1, 2, ...), EvalUpwardHigh 773-854
     for (auto& lv : hfl) {
@@ -794,6 +845,7 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
     }
 
     // ---- HF downward (W = max .. 1), EvalDownwardHigh 996-1040
+    cur_sec = 5;
     for (auto lt = hfl.rbegin(); lt != hfl.rend(); ++lt) {
         double W = lt->first;
         int g0 = (int)pl.tiles.size();
@@ -870,7 +922,11 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
             }
         }
         stage(ST_GEMM, g0, (int)pl.tiles.size(), "hf_l2l");
-        if (W == 1.0 && lf_r0 >= 0) reduce_stage(lf_r0, lf_r1, "lf_reduce");
+        if (W == 1.0 && lf_r0 >= 0) {
+            cur_sec = 6;
+            reduce_stage(lf_r0, lf_r1, "lf_reduce");
+            cur_sec = 5;
+        }
     }
 
     // ---- LF downward (W descending), EvalDownwardLow 493-583
@@ -891,6 +947,7 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
             pl.fsrc.push_back(f);
         }
         spec_max = std::max(spec_max, (int64_t)specidx.size());
+        cur_sec = 3;
         stage(ST_FFTFWD, f0, (int)pl.fsrc.size(), "lf_fft");
         int v0 = (int)pl.vtrg.size();
         for (auto& k : morton_sorted(lt->second)) {
@@ -941,6 +998,7 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
             pl.ktasks.push_back(t);
         }
         stage(ST_KAPPLY, t0, (int)pl.ktasks.size(), "lf_x2chk");
+        cur_sec = 6;
         std::vector<int64_t> all;
         for (auto& k : lt->second) all.push_back(lslot(k));
         int g0 = (int)pl.tiles.size();
@@ -985,49 +1043,63 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
     }
     pl.bufsize[BUF_SPEC] = spec_max * G3;
 
-    // ---- leaf kernel: U list, W list, X list (direct), L2T (eval 550-560, 585-718)
-    int t0 = (int)pl.ktasks.size();
-    for (auto& k : morton_sorted(leaves)) {
-        const Box& b = boxes.at(k);
-        double W = tree.width(k);
-        LfW& w = lfw[W];
-        KTask t{};
-        t.off = ptoff.at(k); t.count = npts(k); t.scale = 1.0;
-        t.obuf = BUF_VAL; t.ooff = t.off; t.mode = 0;
-        t.s0 = (int)pl.ksrcs.size();
-        auto direct = [&](const BoxKey& s) {
-            KSrc q{};
-            q.off = ptoff.at(s); q.count = npts(s); q.scale = 1.0; q.vbuf = BUF_DEN; q.voff = q.off;
-            pl.ksrcs.push_back(q);
-            pl.stats.near_interactions += (int64_t)npts(k) * npts(s);
-        };
-        for (auto& s : b.U) direct(s);
-        for (auto& s : b.W) {
-            const Box& sb = boxes.at(s);
-            if (sb.terminal() && npts(s) < NL) {
-                direct(s);
-            } else {
+    // ---- leaf kernel (eval 550-560, 585-718), in two launches: the near field (U list,
+    // W and X lists evaluated directly from point densities) needs only the densities and
+    // runs on the second stream from the start; the far part (W list through source
+    // equivalent densities, L2T) is added at the end
+    auto leaf_tasks = [&](bool near) {
+        int t0 = (int)pl.ktasks.size();
+        for (auto& k : morton_sorted(leaves)) {
+            const Box& b = boxes.at(k);
+            double W = tree.width(k);
+            LfW& w = lfw[W];
+            KTask t{};
+            t.off = ptoff.at(k); t.count = npts(k); t.scale = 1.0;
+            t.obuf = BUF_VAL; t.ooff = t.off; t.mode = near ? 0 : 1;
+            t.s0 = (int)pl.ksrcs.size();
+            auto direct = [&](const BoxKey& s) {
                 KSrc q{};
-                q.off = w.uep_p; q.count = NL; q.scale = tree.width(s) / W;
-                tree.center(s, q.c);
-                q.vbuf = BUF_LF_UPEQ; q.voff = lslot(s);
+                q.off = ptoff.at(s); q.count = npts(s); q.scale = 1.0; q.vbuf = BUF_DEN; q.voff = q.off;
+                pl.ksrcs.push_back(q);
+                pl.stats.near_interactions += (int64_t)npts(k) * npts(s);
+            };
+            if (near) {
+                for (auto& s : b.U) direct(s);
+                for (auto& s : b.W) {
+                    const Box& sb = boxes.at(s);
+                    if (sb.terminal() && npts(s) < NL) direct(s);
+                }
+                if (npts(k) < NL)
+                    for (auto& s : b.X) direct(s);
+            } else {
+                for (auto& s : b.W) {
+                    const Box& sb = boxes.at(s);
+                    if (sb.terminal() && npts(s) < NL) continue;
+                    KSrc q{};
+                    q.off = w.uep_p; q.count = NL; q.scale = tree.width(s) / W;
+                    tree.center(s, q.c);
+                    q.vbuf = BUF_LF_UPEQ; q.voff = lslot(s);
+                    pl.ksrcs.push_back(q);
+                    pl.stats.leaf_op_interactions += (int64_t)npts(k) * NL;
+                }
+                KSrc q{};
+                q.off = w.ucp_p; q.count = NL; q.scale = 1.0;
+                tree.center(k, q.c);
+                q.vbuf = BUF_LF_DNEQ; q.voff = lslot(k);
                 pl.ksrcs.push_back(q);
                 pl.stats.leaf_op_interactions += (int64_t)npts(k) * NL;
             }
+            t.s1 = (int)pl.ksrcs.size();
+            pl.ktasks.push_back(t);
         }
-        if (npts(k) < NL)
-            for (auto& s : b.X) direct(s);
-        KSrc q{};
-        q.off = w.ucp_p; q.count = NL; q.scale = 1.0;
-        tree.center(k, q.c);
-        q.vbuf = BUF_LF_DNEQ; q.voff = lslot(k);
-        pl.ksrcs.push_back(q);
-        pl.stats.leaf_op_interactions += (int64_t)npts(k) * NL;
-        t.s1 = (int)pl.ksrcs.size();
-        pl.ktasks.push_back(t);
-    }
-    stage(ST_KAPPLY, t0, (int)pl.ktasks.size(), "leaf_near_l2t");
+        stage(ST_KAPPLY, t0, (int)pl.ktasks.size(), near ? "leaf_near" : "leaf_far");
+    };
+    cur_sec = 1;
+    leaf_tasks(true);
+    cur_sec = 6;
+    leaf_tasks(false);
     stage(ST_SCATTER, 0, 0, "scatter");
+    merge_sections();
 
     // ---------------- per-stage algorithmic work (per RHS)
     for (auto& s : pl.stages) {
@@ -1048,6 +1120,20 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
                         for (int h = 0; h < nkb; ++h) k += pl.colB[g.bcol0 + (int64_t)n * nkb + h] >= 0 ? g.kb : 0;
                 }
                 w += 8.0 * rows * k;
+                // executed: per segment (K, or each K block present in some column), every
+                // wave with a column holding it runs ceil(len / 4) k steps of 4 * mb MFMAs
+                const int nkb = g.kb ? g.K / g.kb : 1;
+                const int len = g.kb ? g.kb : g.K;
+                for (int h = 0; h < nkb; ++h) {
+                    int waves = 0;
+                    for (int wv = 0; wv < 4; ++wv) {
+                        bool run = false;
+                        for (int n = tl.n0 + 16 * wv; n < std::min(tl.n0 + 16 * wv + 16, g.ncols) && !run; ++n)
+                            run = g.kb == 0 || pl.colB[g.bcol0 + (int64_t)n * nkb + h] >= 0;
+                        waves += run;
+                    }
+                    s.exec += waves * (double)((len + 3) / 4) * 4 * tl.mb * 2048.0;
+                }
             }
         } else if (s.kind == ST_KAPPLY) {
             for (int t = s.begin; t < s.end; ++t) {

@@ -176,6 +176,14 @@ class Wave3d:
         return [(names.raw[24 * i:24 * i + 24].split(b"\0")[0].decode(), ms[i], work[i], kinds[i] % 16,
                  kinds[i] // 16) for i in range(n)]
 
+    def stage_exec(self):
+        """Per stage, the MFMA flops per RHS the GEMM kernel executes (0 for other stages)."""
+        cnt = ctypes.c_int()
+        check(lib().ddfmm_stage_exec(self._ctx, 0, None, ctypes.byref(cnt)))
+        ex = (ctypes.c_double * cnt.value)()
+        check(lib().ddfmm_stage_exec(self._ctx, cnt.value, ex, ctypes.byref(cnt)))
+        return list(ex)
+
     def close(self):
         if self._ctx is not None:
             lib().ddfmm_destroy(self._ctx)

@@ -94,6 +94,10 @@ int ddfmm_read_buffer(ddfmm_ctx* ctx, int buf, int rhs, int64_t off, int64_t cou
 int ddfmm_set_timing(ddfmm_ctx* ctx, int on);
 int ddfmm_stage_times(ddfmm_ctx* ctx, int cap, char* names /* cap x 24 */, double* ms, double* work,
                       int* kinds, int* count);
+/* Per stage, the MFMA flops per RHS the GEMM kernel executes under its tiling (16-row
+   blocks, 16-column waves, 4-deep k steps, present K blocks only; 0 for other stages):
+   executed / useful work is the padding overhead.  No counterpart in the reference. */
+int ddfmm_stage_exec(ddfmm_ctx* ctx, int cap, double* exec, int* count);
 
 #ifdef __cplusplus
 }

@@ -192,6 +192,13 @@ int ddfmm_setup(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_params* 
             HIPCHK(hipFree(gp));
             HIPCHK(hipFree(jobs));
         }
+        // fused mode: solves folded into the translations by FP64 GEMMs on the table
+        if (!h.setup_stages.empty()) {
+            d.bt.p[BUF_OPMAT] = d.opmat;
+            d.bt.stride[BUF_OPMAT] = 0;
+            for (const Stage& s : h.setup_stages) HIPCHK(launch_setup_gemm(d, s, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
         if (d.mixed) {
             d.opmat32 = c->alloc<float2>(h.opmat_total);
             HIPCHK(launch_to_f32(d.opmat, d.opmat32, h.opmat_total, c->stream));

@@ -8,8 +8,9 @@
 namespace ddfmm {
 
 struct BufTable {
-    void* p[BUF_COUNT];            // double2 (FP64 mode) or float2 (mixed mode) elements
-    int64_t stride[BUF_COUNT];     // elements per rhs slab
+    void* p[BUF_COUNT + 1];        // double2 (FP64 mode) or float2 (mixed mode) elements;
+                                   // p[BUF_OPMAT]: the FP64 operator table (setup GEMMs)
+    int64_t stride[BUF_COUNT + 1]; // elements per rhs slab
 };
 
 struct DevPlan {
@@ -36,6 +37,7 @@ struct DevPlan {
 };
 
 hipError_t launch_stage(const DevPlan& d, const Stage& s, int nrhs, hipStream_t st);
+hipError_t launch_setup_gemm(const DevPlan& d, const Stage& s, hipStream_t st);   // FP64, on the operator table
 hipError_t launch_kgen(const KGen* jobs, int njobs, const double* gp, double2* opmat, int ktype, hipStream_t st);
 hipError_t launch_direct(const double* trg, int M, const double* src, int Ns, const double2* den, int nrhs,
                          double2* out, int ktype, hipStream_t st);

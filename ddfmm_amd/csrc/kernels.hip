@@ -843,6 +843,12 @@ hipError_t launch_stage(const DevPlan& d, const Stage& s, int nrhs, hipStream_t 
     return d.mixed ? launch_stage_t<float2>(d, s, nrhs, st) : launch_stage_t<double2>(d, s, nrhs, st);
 }
 
+hipError_t launch_setup_gemm(const DevPlan& d, const Stage& s, hipStream_t st) {
+    if (s.kind != ST_GEMM || s.mb < 1 || s.mb > GEMM_MB) return hipErrorInvalidValue;
+    launch_gemm<double2, double2, double>(s.mb, s.end - s.begin, 1, st, d, d.opmat, s.begin);
+    return hipGetLastError();
+}
+
 hipError_t launch_kgen(const KGen* jobs, int njobs, const double* gp, double2* opmat, int ktype, hipStream_t st) {
     for (int j0 = 0; j0 < njobs; j0 += 65536) {
         int n = std::min(65536, njobs - j0);

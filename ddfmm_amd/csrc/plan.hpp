@@ -24,6 +24,8 @@ enum Buf : int {
     BUF_RED,        // HF M2L/L2L contributions of one level, one slot per (target, source)
     BUF_COUNT
 };
+// setup-time GEMMs read and write the operator table itself (FP64), as buffer BUF_OPMAT
+constexpr int BUF_OPMAT = BUF_COUNT;
 
 // kernel-apply: out(target_i) (+)= sum_src sum_j G(t_i, s_j) v_j
 // point coordinates: pts[off + i] * scale + c

@@ -100,8 +100,13 @@ struct Builder {
     }
     // B offsets: ncols entries (kb = 0, contiguous columns) or ncols x ceil(K/kb) block
     // offsets (kb > 0, -1 = zero block); C offsets: ncols entries
+    int64_t reserve(int64_t n) {        // generated-part space written by a setup GEMM
+        int64_t off = gen_cursor;
+        gen_cursor += n;
+        return off;
+    }
     int add_group(GGroup g, const std::vector<int64_t>& bc, const std::vector<int64_t>& cc,
-                  double* flops, int kb = 0, const std::vector<int64_t>& ablk = {}) {
+                  double* flops, int kb = 0, const std::vector<int64_t>& ablk = {}, bool count = true) {
         if (cc.empty()) return -1;
         if (kb > 0 && (g.K % kb != 0 || g.K / kb > 8))
             throw Error("internal: K must split into at most 8 equal blocks");
@@ -130,7 +135,7 @@ struct Builder {
             for (int64_t v : bc) present += v >= 0;
             f = 8.0 * g.M * kb * (double)present;
         }
-        pl.stats.gemm_flops += f;
+        if (count) pl.stats.gemm_flops += f;
         pl.group_flops[gi] = f;
         if (flops) *flops += f;
         return gi;
@@ -177,6 +182,21 @@ struct Builder {
         pl.stages.swap(out);
         stage_sec.clear();
     }
+    // one launch per tile height (a kernel instance each): tiles sorted by height, the
+    // tiles of a group kept together
+    void push_gemm_stages(std::vector<Stage>& dst, const Stage& s) {
+        std::stable_sort(pl.tiles.begin() + s.begin, pl.tiles.begin() + s.end,
+                         [](const GTile& x, const GTile& y) { return x.mb < y.mb; });
+        for (int t = s.begin; t < s.end;) {
+            int u = t;
+            while (u < s.end && pl.tiles[u].mb == pl.tiles[t].mb) ++u;
+            if (pl.tiles[t].mb < 1 || pl.tiles[t].mb > GEMM_MB) throw Error("internal: tile height out of range");
+            Stage q = s;
+            q.begin = t; q.end = u; q.mb = pl.tiles[t].mb;
+            dst.push_back(q);
+            t = u;
+        }
+    }
     void stage_emit(int kind, int b, int e, const char* name, int buf, int64_t count) {
         if (kind != ST_ZERO && kind != ST_GATHER && kind != ST_SCATTER && e <= b) return;
         Stage s;
@@ -189,19 +209,7 @@ struct Builder {
                   nm.find("hf_up_e") != std::string::npos || nm.find("hf_dn_e") != std::string::npos)
                      ? PREC_SOLVE : PREC_TRANSLATION;
         if (kind == ST_GEMM) {
-            // one launch per tile height (a kernel instance each): tiles sorted by height,
-            // the tiles of a group kept together
-            std::stable_sort(pl.tiles.begin() + b, pl.tiles.begin() + e,
-                             [](const GTile& x, const GTile& y) { return x.mb < y.mb; });
-            for (int t = b; t < e;) {
-                int u = t;
-                while (u < e && pl.tiles[u].mb == pl.tiles[t].mb) ++u;
-                if (pl.tiles[t].mb < 1 || pl.tiles[t].mb > GEMM_MB) throw Error("internal: tile height out of range");
-                Stage q = s;
-                q.begin = t; q.end = u; q.mb = pl.tiles[t].mb;
-                pl.stages.push_back(q);
-                t = u;
-            }
+            push_gemm_stages(pl.stages, s);
             return;
         }
         pl.stages.push_back(s);
@@ -253,12 +261,22 @@ struct Builder {
             need(t.mb >= 1 && t.mb <= GEMM_MB && t.m0 >= 0 && t.m0 % 16 == 0 && t.m0 < g.M &&
                  t.m0 + 16 * (t.mb - 1) < g.M && t.n0 >= 0 && t.n0 < g.ncols, "tile index");
         }
+        auto bsize = [&](int b) { return b == BUF_OPMAT ? pl.opmat_total : pl.bufsize[b]; };
+        for (const Stage& st : pl.setup_stages) {
+            need(st.kind == ST_GEMM && st.begin >= 0 && st.begin <= st.end && st.end <= (int64_t)pl.tiles.size(),
+                 "setup stage");
+            for (int t = st.begin; t < st.end; ++t) {
+                need(pl.tiles[t].mb == st.mb, "setup tile height");
+                const GGroup& g = pl.groups[pl.tiles[t].g];
+                need(g.bbuf == BUF_OPMAT && g.cbuf == BUF_OPMAT, "setup buffers");
+            }
+        }
         for (const GGroup& g : pl.groups) {
-            need(g.bbuf >= 0 && g.bbuf < BUF_COUNT && g.cbuf >= 0 && g.cbuf < BUF_COUNT, "gemm buffers");
+            need(g.bbuf >= 0 && g.bbuf <= BUF_OPMAT && g.cbuf >= 0 && g.cbuf <= BUF_OPMAT, "gemm buffers");
             need(g.col0 >= 0 && g.col0 + (int64_t)g.ncols <= (int64_t)pl.colC.size(), "gemm C columns");
             for (int n = 0; n < g.ncols; ++n) {
                 const int64_t c = pl.colC[g.col0 + n];
-                need(c >= 0 && c + g.M <= pl.bufsize[g.cbuf], "gemm C range");
+                need(c >= 0 && c + g.M <= bsize(g.cbuf), "gemm C range");
             }
             if (g.soff >= 0) need(g.soff + g.M <= pl.opmat_total, "gemm row scale");
             auto a_ok = [&](int64_t aoff, int kcols) {
@@ -271,7 +289,7 @@ struct Builder {
                 need(g.bcol0 + (int64_t)g.ncols <= (int64_t)pl.colB.size(), "gemm B columns");
                 for (int n = 0; n < g.ncols; ++n) {
                     const int64_t b = pl.colB[g.bcol0 + n];
-                    need(b >= 0 && b + g.K <= pl.bufsize[g.bbuf], "gemm B range");
+                    need(b >= 0 && b + g.K <= bsize(g.bbuf), "gemm B range");
                 }
             } else {
                 const int nkb = g.K / g.kb;
@@ -281,7 +299,7 @@ struct Builder {
                 need(g.bcol0 + (int64_t)g.ncols * nkb <= (int64_t)pl.colB.size(), "gemm B blocks");
                 for (int64_t e = 0; e < (int64_t)g.ncols * nkb; ++e) {
                     const int64_t b = pl.colB[g.bcol0 + e];
-                    need(b >= -1 && b + g.kb <= pl.bufsize[g.bbuf], "gemm B block range");
+                    need(b >= -1 && b + g.kb <= bsize(g.bbuf), "gemm B block range");
                 }
             }
         }
@@ -328,6 +346,10 @@ struct HfDir {          // one (width, direction) group
     int64_t ue2uc[8];   // generated matrices (relative offsets)
     int64_t de2dc[8];   // W = 1 only: G(uep_LF(W/2) + s_c, -ucp) (not a transpose of ue2uc)
     int64_t e1, e2, e3; // fused: e1 = E1 E2 E3 (r1 x r2); factored: E1, E2, E3
+    // fused mode: solves folded into the translations (relative generated offsets)
+    int64_t F = -1;     // F_c = (E1 E2 E3) ue2uc_c, r1 x r1c each, contiguous over c
+    int64_t D[8];       // W = 1: D_c = de2dc_c (E1 E2 E3)^T, r1c x r1
+    int64_t eT = -1;    // W = 1: (E1 E2 E3)^T (r2 x r1), uploaded
     std::vector<BoxKey> out, in;
 };
 
@@ -452,7 +474,8 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
     std::map<double, std::vector<HfDir>> hfl;
     std::map<std::pair<BoxKey, Index3>, int64_t> outslot, inslot, oscr, iscr;
     int64_t hf_up_size = 0, hf_dn_size = 0, scr_size = 0;
-    std::map<std::pair<double, Index3>, std::pair<int64_t, int64_t>> hf_entry_off;   // (E1..E3) cache
+    struct EOff { int64_t e1, e2, eT; };
+    std::map<std::pair<double, Index3>, EOff> hf_entry_off;   // E1..E3 (or their product) per entry
     for (auto& kv : hdmap) {
         const Index3& d = kv.first;
         double W = double(d.linfty() / prm.NPQ);
@@ -489,20 +512,30 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
             auto key = std::make_pair(W, h.srt);
             auto it = hf_entry_off.find(key);
             if (it == hf_entry_off.end()) {
-                int64_t e1, e2;
+                EOff o{-1, -1, -1};
                 if (prm.fuse_solves) {
-                    // E1 * E2 * E3 (r1 x r2) as one FP64 matrix
-                    e1 = e2 = upload(matmul(matmul(h.he->uc2ue[0], h.he->uc2ue[1]), h.he->uc2ue[2]));
+                    // E1 * E2 * E3 (r1 x r2) as one FP64 matrix, and at W = 1 its transpose
+                    const CMat Ef = matmul(matmul(h.he->uc2ue[0], h.he->uc2ue[1]), h.he->uc2ue[2]);
+                    o.e1 = o.e2 = upload(Ef);
+                    if (W == 1.0) {
+                        CMat T;
+                        T.m = Ef.n; T.n = Ef.m;
+                        T.a.resize(Ef.a.size());
+                        for (int j = 0; j < Ef.n; ++j)
+                            for (int i = 0; i < Ef.m; ++i) T.a[j + (size_t)i * T.m] = Ef.a[i + (size_t)j * Ef.m];
+                        o.eT = upload(T);
+                    }
                 } else {
-                    e1 = upload(h.he->uc2ue[0]);
-                    e2 = upload(h.he->uc2ue[1]);
+                    o.e1 = upload(h.he->uc2ue[0]);
+                    o.e2 = upload(h.he->uc2ue[1]);
                     upload(h.he->uc2ue[2]);
                 }
-                it = hf_entry_off.emplace(key, std::make_pair(e1, e2)).first;
+                it = hf_entry_off.emplace(key, o).first;
             }
-            h.e1 = it->second.first;
-            h.e2 = it->second.second;
+            h.e1 = it->second.e1;
+            h.e2 = it->second.e2;
             h.e3 = h.e2 + (int64_t)h.r1 * h.r2;
+            h.eT = it->second.eT;
             // shuffled point sets and child kernels (UpwardHighFetch, mlib3d.cpp:176-224)
             std::vector<double> uep = Mlib::shuffled(h.prm, h.sgn, h.he->uep);
             std::vector<double> ucp = Mlib::shuffled(h.prm, h.sgn, h.he->ucp);
@@ -535,6 +568,10 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
                     double ms[3] = {-s[0], -s[1], -s[2]};
                     h.de2dc[c] = gen(h.uepc_g, h.r1c, ms, 1, h.ucp_g, h.r2, zero);
                 }
+            }
+            if (prm.fuse_solves) {
+                h.F = reserve((int64_t)h.r1 * 8 * h.r1c);
+                for (int c = 0; c < 8; ++c) h.D[c] = W == 1.0 ? reserve((int64_t)h.r1c * h.r1) : -1;
             }
         }
         scr_size = std::max(scr_size, std::max(lvl_o, lvl_i));
@@ -679,6 +716,46 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
     pl.stats.opmat_elems = pl.opmat_total;
     pl.stats.opmat_generated = gen_cursor;
 
+    // ---------------- fused mode: solves folded into the translations, by FP64 GEMMs on
+    // the operator table at setup (after the generation of the child kernels):
+    //   upward    F_c = (E1 E2 E3) ue2uc_c: child equivalents -> parent equivalents, and
+    //             (W > 1) transposed for the downward pass: de2dc_c (E1 E2 E3)^T = F_{7-c}^T
+    //   W = 1     D_c = de2dc_c (E1 E2 E3)^T: in-slot check values -> LF child check values
+    if (prm.fuse_solves) {
+        const int g0 = (int)pl.tiles.size();
+        for (auto& lv : hfl)
+            for (auto& h : lv.second) {
+                std::vector<int64_t> bc, cc;
+                for (int j = 0; j < 8 * h.r1c; ++j) {
+                    bc.push_back(G_(h.ue2uc[0]) + (int64_t)j * h.r2);
+                    cc.push_back(G_(h.F) + (int64_t)j * h.r1);
+                }
+                GGroup g{};
+                g.aoff = h.e1; g.soff = -1; g.M = h.r1; g.K = h.r2; g.lda = h.r1; g.transA = 0; g.mode = 0;
+                g.bbuf = BUF_OPMAT; g.cbuf = BUF_OPMAT;
+                add_group(g, bc, cc, nullptr, 0, {}, false);
+                if (lv.first != 1.0) continue;
+                for (int c = 0; c < 8; ++c) {
+                    std::vector<int64_t> bd, cd;
+                    for (int j = 0; j < h.r1; ++j) {
+                        bd.push_back(h.eT + (int64_t)j * h.r2);
+                        cd.push_back(G_(h.D[c]) + (int64_t)j * h.r1c);
+                    }
+                    GGroup q{};
+                    q.aoff = G_(h.de2dc[c]); q.soff = -1; q.M = h.r1c; q.K = h.r2; q.lda = h.r1c; q.transA = 0;
+                    q.mode = 0; q.bbuf = BUF_OPMAT; q.cbuf = BUF_OPMAT;
+                    add_group(q, bd, cd, nullptr, 0, {}, false);
+                }
+            }
+        if ((int)pl.tiles.size() > g0) {
+            Stage s;
+            std::memset(&s, 0, sizeof(s));
+            s.kind = ST_GEMM; s.begin = g0; s.end = (int)pl.tiles.size(); s.prec = PREC_SOLVE;
+            std::strncpy(s.name, "setup_fuse", sizeof(s.name) - 1);
+            push_gemm_stages(pl.setup_stages, s);
+        }
+    }
+
     // ---------------- buffers (elements per rhs)
     pl.bufsize[BUF_DEN] = N;
     pl.bufsize[BUF_VAL] = N;
@@ -688,7 +765,7 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
     pl.bufsize[BUF_LF_DNEQ] = nlf * NL;
     pl.bufsize[BUF_HF_UP] = hf_up_size;
     pl.bufsize[BUF_HF_DN] = hf_dn_size;
-    pl.bufsize[BUF_HF_A] = scr_size;
+    pl.bufsize[BUF_HF_A] = prm.fuse_solves ? 0 : scr_size;
     pl.bufsize[BUF_HF_B] = prm.fuse_solves ? 0 : scr_size;
     pl.bufsize[BUF_RED] = red_size;
 
@@ -721,8 +798,9 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
         int g0 = (int)pl.tiles.size();
         if (w.has_children) {
             // one GEMM: [ue2uc_0 .. ue2uc_7] (NL x 8 NL) times the children stacked along K;
-            // columns sorted by child-occupancy mask so that a tile's union of present
-            // children (the K blocks it multiplies) stays small
+            // columns sorted by child-occupancy mask (Gray-code order: neighbouring masks
+            // differ in one child) so that a wave's union of present children (the K blocks
+            // it multiplies) stays small
             std::vector<std::pair<int, BoxKey>> cols;
             for (auto& k : kv.second) {
                 if (boxes.at(k).terminal()) continue;
@@ -732,7 +810,9 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
                 cols.push_back({mask, k});
             }
             std::stable_sort(cols.begin(), cols.end(),
-                             [](const std::pair<int, BoxKey>& a, const std::pair<int, BoxKey>& b) { return a.first < b.first; });
+                             [](const std::pair<int, BoxKey>& a, const std::pair<int, BoxKey>& b) {
+                                 return (a.first ^ (a.first >> 1)) < (b.first ^ (b.first >> 1));
+                             });
             std::vector<int64_t> bb, cc, ab;
             for (auto& mk : cols) {
                 for (int c = 0; c < 8; ++c) {
@@ -790,7 +870,9 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
                 cols.push_back({mask, k});
             }
             std::stable_sort(cols.begin(), cols.end(),
-                             [](const std::pair<int, BoxKey>& a, const std::pair<int, BoxKey>& b) { return a.first < b.first; });
+                             [](const std::pair<int, BoxKey>& a, const std::pair<int, BoxKey>& b) {
+                                 return (a.first ^ (a.first >> 1)) < (b.first ^ (b.first >> 1));
+                             });
             std::vector<int64_t> bb, cc, ab;
             for (auto& mk : cols) {
                 for (int c = 0; c < 8; ++c) {
@@ -800,28 +882,22 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
                 }
                 cc.push_back(oscr.at({mk.second, h.dir}));
             }
-            for (int c = 0; c < 8; ++c) ab.push_back(G_(h.ue2uc[c]));
             GGroup g{};
-            g.aoff = -1; g.soff = -1; g.M = h.r2; g.K = 8 * h.r1c; g.lda = h.r2; g.transA = 0;
-            g.bbuf = W == 1.0 ? BUF_LF_UPEQ : BUF_HF_UP; g.cbuf = BUF_HF_A; g.mode = 0;
+            g.aoff = -1; g.soff = -1; g.K = 8 * h.r1c; g.transA = 0; g.mode = 0;
+            g.bbuf = W == 1.0 ? BUF_LF_UPEQ : BUF_HF_UP;
+            if (prm.fuse_solves) {
+                // equivalents directly: F_c (r1 x r1c) into the out-slots
+                for (int c = 0; c < 8; ++c) ab.push_back(G_(h.F) + (int64_t)c * h.r1 * h.r1c);
+                for (size_t j = 0; j < cols.size(); ++j) cc[j] = outslot.at({cols[j].second, h.dir});
+                g.M = h.r1; g.lda = h.r1; g.cbuf = BUF_HF_UP;
+            } else {
+                for (int c = 0; c < 8; ++c) ab.push_back(G_(h.ue2uc[c]));
+                g.M = h.r2; g.lda = h.r2; g.cbuf = BUF_HF_A;
+            }
             add_group(g, bb, cc, nullptr, h.r1c, ab);
         }
         stage(ST_GEMM, g0, (int)pl.tiles.size(), "hf_u2u");
-        if (prm.fuse_solves) {
-            g0 = (int)pl.tiles.size();
-            for (auto& h : lv.second) {
-                std::vector<int64_t> bc, cc;
-                for (auto& k : h.out) {
-                    bc.push_back(oscr.at({k, h.dir}));
-                    cc.push_back(outslot.at({k, h.dir}));
-                }
-                GGroup g{};
-                g.aoff = h.e1; g.soff = -1; g.M = h.r1; g.K = h.r2; g.lda = h.r1; g.transA = 0; g.mode = 0;
-                g.bbuf = BUF_HF_A; g.cbuf = BUF_HF_UP;
-                add_group(g, bc, cc, nullptr);
-            }
-            stage(ST_GEMM, g0, (int)pl.tiles.size(), "hf_up_e");
-        } else {
+        if (!prm.fuse_solves) {
             // E3, then E2, then E1 (EvalUpwardHigh, wave3d_eval.cpp:833-852)
             const char* nm[3] = {"hf_up_e3", "hf_up_e2", "hf_up_e1"};
             for (int step = 0; step < 3; ++step) {
@@ -862,22 +938,7 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
         }
         stage(ST_GEMM, g0, (int)pl.tiles.size(), "hf_m2l");
         reduce_stage(hf_region.at(W).first, hf_region.at(W).second, "hf_reduce");
-        if (prm.fuse_solves) {
-            g0 = (int)pl.tiles.size();
-            for (auto& h : lt->second) {
-                std::vector<int64_t> bc, cc;
-                for (auto& k : h.in) {
-                    bc.push_back(inslot.at({k, h.dir}));
-                    cc.push_back(iscr.at({k, h.dir}));
-                }
-                // (E1 E2 E3)^T = E3^T E2^T E1^T (DownwardHighFetch transposes, mlib3d.cpp:247-252)
-                GGroup g{};
-                g.aoff = h.e1; g.soff = -1; g.M = h.r2; g.K = h.r1; g.lda = h.r1; g.transA = 1; g.mode = 0;
-                g.bbuf = BUF_HF_DN; g.cbuf = BUF_HF_A;
-                add_group(g, bc, cc, nullptr);
-            }
-            stage(ST_GEMM, g0, (int)pl.tiles.size(), "hf_dn_et");
-        } else {
+        if (!prm.fuse_solves) {
             // E1^T, then E2^T, then E3^T (HighFrequencyL2L, wave3d_eval.cpp:941-949)
             const char* nm[3] = {"hf_dn_e1t", "hf_dn_e2t", "hf_dn_e3t"};
             for (int step = 0; step < 3; ++step) {
@@ -905,19 +966,31 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
                 for (auto& k : h.in) {
                     BoxKey ck = Tree::child_key(k, c);
                     if (!child_present(ck)) continue;
-                    bc.push_back(iscr.at({k, h.dir}));
+                    bc.push_back(prm.fuse_solves ? inslot.at({k, h.dir}) : iscr.at({k, h.dir}));
                     cc.push_back(slot(l2l_target(h, ck)));
                 }
                 // W > 1: de2dc(c) = G(-uepc + s_c, -ucp) = ue2uc(7 - c)^T (mlib3d.cpp:227-280);
-                // W = 1: explicit matrix (r1c x r2)
+                // W = 1: explicit matrix (r1c x r2).  Fused mode: composed with the downward
+                // solve (E1 E2 E3)^T, from the in-slot check values: F_{7-c}^T, or D_c at W = 1
                 GGroup g{};
-                if (W == 1.0) {
-                    g.aoff = G_(h.de2dc[c]); g.M = h.r1c; g.K = h.r2; g.lda = h.r1c; g.transA = 0;
+                if (prm.fuse_solves) {
+                    if (W == 1.0) {
+                        g.aoff = G_(h.D[c]); g.M = h.r1c; g.K = h.r1; g.lda = h.r1c; g.transA = 0;
+                    } else {
+                        g.aoff = G_(h.F) + (int64_t)(7 - c) * h.r1 * h.r1c;
+                        g.M = h.r1c; g.K = h.r1; g.lda = h.r1; g.transA = 1;
+                    }
+                    g.bbuf = BUF_HF_DN;
                 } else {
-                    g.aoff = G_(h.ue2uc[7 - c]); g.M = h.r1c; g.K = h.r2; g.lda = h.r2; g.transA = 1;
+                    if (W == 1.0) {
+                        g.aoff = G_(h.de2dc[c]); g.M = h.r1c; g.K = h.r2; g.lda = h.r1c; g.transA = 0;
+                    } else {
+                        g.aoff = G_(h.ue2uc[7 - c]); g.M = h.r1c; g.K = h.r2; g.lda = h.r2; g.transA = 1;
+                    }
+                    g.bbuf = BUF_HF_A;
                 }
                 g.soff = -1;
-                g.bbuf = BUF_HF_A; g.cbuf = BUF_RED; g.mode = 0;
+                g.cbuf = BUF_RED; g.mode = 0;
                 add_group(g, bc, cc, nullptr);
             }
         }

@@ -45,6 +45,7 @@ struct HostPlan {
     std::vector<VPair> vpair;
     std::vector<RTarget> rtrg;
     std::vector<Stage> stages;
+    std::vector<Stage> setup_stages;       // FP64 GEMMs on the operator table, run once after generation
     int64_t bufsize[BUF_COUNT] = {0};      // elements per rhs
     PlanStats stats;
     std::string dump;                      // canonical text of the lists (debug/parity)

@@ -108,7 +108,8 @@ def _host_stages(oplib, pos, K, precision):
 def test_stage_precision_classes_and_work(oplib, precision):
     """Solve stages (check -> equivalent: LF pinv, HF E1 E2 E3 chains) carry the solve class
     that the mixed mode runs in FP64; every GEMM launch (one per tile height) keeps it; the
-    per-launch useful flops add up to the planner's GEMM total."""
+    per-launch useful flops add up to the planner's GEMM total.  In the mixed mode the HF
+    solves are folded into the translations at setup, so no HF solve stage remains."""
     pos = geometry.sphere(20000, 8.0, 0)
     stages, st = _host_stages(oplib, pos, 8.0, precision)
     ST_GEMM, ST_REDUCE = 4, 7
@@ -119,7 +120,8 @@ def test_stage_precision_classes_and_work(oplib, precision):
         assert prec == (1 if any(t in name for t in solve_names) else 0), name
     names = {s[0] for s in gemm}
     if precision == 1:
-        assert {"lf_uc2ue", "lf_dc2de", "hf_up_e", "hf_dn_et"} <= names
+        assert {"lf_uc2ue", "lf_dc2de", "hf_u2u", "hf_l2l"} <= names
+        assert not any(n.startswith("hf_up_e") or n.startswith("hf_dn_e") for n in names)
     else:
         assert {"lf_uc2ue_1", "lf_uc2ue_2", "hf_up_e3", "hf_dn_e1t"} <= names
     total = sum(s[1] for s in gemm)

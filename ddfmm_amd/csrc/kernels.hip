@@ -58,19 +58,17 @@ __device__ __forceinline__ double2 green64(double dx, double dy, double dz, int 
     return make_double2(c, s);
 }
 
-// FP32 G(r) from FP64 differences: v_sqrt_f32, v_rcp_f32, v_sin_f32/v_cos_f32 on the
-// phase r - rint(r) in revolutions.
+// FP32 G(r) from FP64 differences: 1/r from v_rsq_f32, r = r^2 / r, and v_sin_f32 /
+// v_cos_f32 on the phase r - rint(r) in revolutions (three transcendental operations).
 __device__ __forceinline__ float2 green32(double dx, double dy, double dz, int ktype) {
     float fx = (float)dx, fy = (float)dy, fz = (float)dz;
     float r2 = fx * fx + fy * fy + fz * fz;
     r2 = r2 < 1e-16f ? 1.0f : r2;
-    float r = __builtin_amdgcn_sqrtf(r2);
+    const float ir = __builtin_amdgcn_rsqf(r2);
+    const float r = r2 * ir;
     float t = r - __builtin_rintf(r);
     float s = __builtin_amdgcn_sinf(t), c = __builtin_amdgcn_cosf(t);
-    if (ktype == 0) {
-        float ir = __builtin_amdgcn_rcpf(r);
-        return make_float2(c * ir, s * ir);
-    }
+    if (ktype == 0) return make_float2(c * ir, s * ir);
     return make_float2(c, s);
 }
 

@@ -141,9 +141,11 @@ struct Builder {
         return gi;
     }
     // Stages are emitted in dependency order per section and merged at the end:
-    //   0 gather | fork 0 | 1 near field (stream 1) | 2 LF upward | fork 1 |
-    //   3 FFT, V list, X list per LF width (stream 1) | 4 HF upward | 5 HF downward |
-    //   join | 6 W=1 -> LF reduction, LF downward, leaf far field, scatter
+    //   0 gather | fork 0 | 1 (stream 1, unused) | 2 LF upward | fork 1 |
+    //   3 FFT, V list, X list per LF width, near field (stream 1) | 4 HF upward |
+    //   5 HF downward | join | 6 W=1 -> LF reduction, LF downward, leaf far field, scatter
+    // (the near field waits for the LF upward pass: running beside it slows that short
+    // chain more than it gains)
     int cur_sec = 0;
     std::vector<int> stage_sec;
     void stage(int kind, int b, int e, const char* name, int buf = -1, int64_t count = 0) {
@@ -1167,7 +1169,7 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
         }
         stage(ST_KAPPLY, t0, (int)pl.ktasks.size(), near ? "leaf_near" : "leaf_far");
     };
-    cur_sec = 1;
+    cur_sec = 3;
     leaf_tasks(true);
     cur_sec = 6;
     leaf_tasks(false);

@@ -276,7 +276,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"mixed": "fp32+fp64 (FP32 translations/near field, FP64 solves)", "fp64": "fp64",
+        "dtype": {"mixed": "fp32+fp64 (FP32: translations with the HF solves folded in, M2L, near field, V list; FP64: LF check-to-equivalent solves)", "fp64": "fp64",
                   "fp64_fused": "fp64 (fused solves)"}[args.precision],
         "data": "synthetic",
         "config": {"workload": args.workload, "n_points": N, "nrhs": nrhs, "kernel": "exp(ikr)/r" if args.kernel == 0

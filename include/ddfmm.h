@@ -68,7 +68,9 @@ int ddfmm_setup(ddfmm_ctx* ctx, const double* pos, int64_t n, const ddfmm_params
 int ddfmm_eval(ddfmm_ctx* ctx, const double* den, double* val, int nrhs);
 
 /* Same with device-resident buffers on `stream` (0 = the context's stream).
-   Asynchronous with respect to the host. */
+   Asynchronous with respect to the host.  Part of the work runs on a second stream of
+   the context that is forked from and joined back into `stream` with events, so work
+   queued on `stream` after this call sees the complete result. */
 int ddfmm_eval_device(ddfmm_ctx* ctx, const void* den_dev, void* val_dev, int nrhs, void* stream);
 
 /* Direct FP64 sum at targets (Wave3d::check's oracle sum, wave3d_check.cpp:66-76):

@@ -215,14 +215,15 @@ template <> struct Mfma<double> {
 
 constexpr int MAXKB = 8;           // at most 8 K blocks (children of a box)
 
-// LDS layouts (conflict-free fragment reads for wave64 MFMA 16x16x4):
-//   A slab [k][m], row stride SA = 16 MB words, or 16 MB + 16 for even MB, so that
-//   SA = 16 mod 32: the two k rows of a 32-lane f32 read land in disjoint bank halves
-//   (f64 reads serve 16 lanes per cycle, one k row);
-//   B slab [n][k], row stride SB: f32 18 (even/odd rows split the banks), f64 17 (odd).
+// LDS layouts: complex values interleaved (one 64-bit (f32) or 128-bit (f64) access per
+// element); every 16-lane (f32) or 8-lane (f64) group of a fragment read touches one
+// contiguous 128-byte row segment or distinct banks:
+//   A slab [k][m], row stride SA = 16 MB + 4 (f32) or + 2 (f64) complex: the 4 k rows
+//   written by 4 neighbouring lanes in the transposed-A store land in distinct banks;
+//   B slab [n][k], row stride SB = 17 (odd: 16 columns at one k in distinct banks).
 template <class R> struct Lds;
-template <> struct Lds<float> { static constexpr int SB = 18; };
-template <> struct Lds<double> { static constexpr int SB = 17; };
+template <> struct Lds<float> { static constexpr int SAPAD = 4; using C = float2; };
+template <> struct Lds<double> { static constexpr int SAPAD = 2; using C = double2; };
 
 // C[:, ccol] (=|+=) diag(scale) * op(A) * B[:, bcol] on MFMA 16x16x4 (f32 or f64).
 // A tile covers MB row blocks of 16 (the group's rows split evenly over its tiles; tiles
@@ -239,7 +240,8 @@ __global__ __launch_bounds__(256) void gemm_kernel(
     const TA* __restrict__ opm, BufTable bt, int tile0) {
     using M = Mfma<R>;
     using T = typename Real<V>::T;
-    constexpr int KS = 16, TM = 16 * MB, SA = (MB & 1) ? TM : TM + 16, SB = Lds<R>::SB;
+    using RC = typename Lds<R>::C;
+    constexpr int KS = 16, TM = 16 * MB, SA = TM + Lds<R>::SAPAD, SB = 17;
     constexpr int NA = TM * KS / 256, NB = GEMM_TN * KS / 256;   // elements per thread and slab
     const GTile tl = tiles[tile0 + xcd_item(blockIdx.x, gridDim.x)];
     const GGroup g = groups[tl.g];
@@ -249,8 +251,8 @@ __global__ __launch_bounds__(256) void gemm_kernel(
     // f32: two LDS slab buffers (one barrier per slab); f64 slabs are twice the size and
     // take twice the MFMA time per k, so one buffer and two barriers per slab
     constexpr int NBUF = sizeof(R) == 4 ? 2 : 1;
-    __shared__ R Ar[NBUF][KS * SA], Ai[NBUF][KS * SA];
-    __shared__ R Br[NBUF][GEMM_TN * SB], Bi[NBUF][GEMM_TN * SB];
+    __shared__ RC As[NBUF][KS * SA];
+    __shared__ RC Bs[NBUF][GEMM_TN * SB];
     __shared__ int64_t bo[MAXKB][GEMM_TN];   // B offset of (block, column), -1 = absent
     __shared__ int64_t ao[MAXKB];            // A offset of each active segment
     __shared__ int used[MAXKB][4], act[MAXKB], wm[MAXKB], nact_s;
@@ -360,13 +362,17 @@ __global__ __launch_bounds__(256) void gemm_kernel(
     auto store = [&](int q) {
 #pragma unroll
         for (int l = 0; l < NA; ++l) {
-            Ar[q][alds[l]] = (R)ra[l].x;
-            Ai[q][alds[l]] = (R)ra[l].y;
+            RC v;
+            v.x = (R)ra[l].x;
+            v.y = (R)ra[l].y;
+            As[q][alds[l]] = v;
         }
 #pragma unroll
         for (int l = 0; l < NB; ++l) {
-            Br[q][blds[l]] = (R)rb[l].x;
-            Bi[q][blds[l]] = (R)rb[l].y;
+            RC v;
+            v.x = (R)rb[l].x;
+            v.y = (R)rb[l].y;
+            Bs[q][blds[l]] = v;
         }
     };
 
@@ -386,12 +392,14 @@ __global__ __launch_bounds__(256) void gemm_kernel(
         if (!((((g.kb ? wm[i] : wm0)) >> wave) & 1)) return;
         const int nsteps = (min(KS, kb - k0) + 3) >> 2;
         for (int s = 0; s < nsteps; ++s) {
-            const R br = Br[q][bb + 4 * s], bi = Bi[q][bb + 4 * s], mbi = -bi;
+            const RC bv = Bs[q][bb + 4 * s];
+            const R br = bv.x, bi = bv.y, mbi = -bi;
             R ar[MB], ai[MB];
 #pragma unroll
             for (int a = 0; a < MB; ++a) {
-                ar[a] = Ar[q][aa + 4 * s * SA + 16 * a];
-                ai[a] = Ai[q][aa + 4 * s * SA + 16 * a];
+                const RC av = As[q][aa + 4 * s * SA + 16 * a];
+                ar[a] = av.x;
+                ai[a] = av.y;
             }
 #pragma unroll
             for (int a = 0; a < MB; ++a) cr[a] = M::op(ar[a], br, cr[a]);

@@ -207,10 +207,11 @@ def main():
         f"{k}={v['ms']:.3f}/{v['launches']}" for k, v in sorted(per.items(), key=lambda kv: -kv[1]["ms"])))
     mixed = args.precision == "mixed"
     esz = 8 if mixed else 16
+    vt = "float2" if mixed else "double2"
     kname = {"gemm_translation": "gemm_kernel<float2,float2,float>" if mixed else "gemm_kernel<double2,double2,double>",
              "gemm_solve": "gemm_kernel<double2,float2,double>" if mixed else "gemm_kernel<double2,double2,double>",
-             "kapply": "kapply_kernel", "vlist": "vlist_kernel<12>", "fft_fwd": "fft_fwd_kernel<12>",
-             "reduce": "reduce_kernel"}.get(dom, dom)
+             "kapply": f"kapply_kernel<{vt},{1 if nrhs == 1 else 4}>", "vlist": f"vlist_acc_kernel<{vt}>",
+             "fft_fwd": f"fft_fwd_kernel<{vt}>", "reduce": f"reduce_kernel<{vt}>"}.get(dom, dom)
     if dom in ("gemm_translation", "gemm_solve"):
         peak = PEAK["fp32"] if (mixed and dom == "gemm_translation") else PEAK["fp64"]
         roof = {"bound": "mfma", "achieved": per[dom]["work"] / t_dom / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s"}

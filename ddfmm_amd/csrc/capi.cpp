@@ -170,7 +170,8 @@ int ddfmm_setup(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_params* 
         d.ktasks = c->upload(h.ktasks); d.ksrcs = c->upload(h.ksrcs);
         d.groups = c->upload(h.groups); d.tiles = c->upload(h.tiles);
         d.colB = c->upload(h.colB); d.colC = c->upload(h.colC); d.colA = c->upload(h.colA);
-        d.fsrc = c->upload(h.fsrc); d.vtrg = c->upload(h.vtrg); d.vpair = c->upload(h.vpair);
+        d.fsrc = c->upload(h.fsrc); d.vtrg = c->upload(h.vtrg);
+        d.vchunk = c->upload(h.vchunk); d.vsrc = c->upload(h.vsrc); d.vpk = c->upload(h.vpk);
         d.rtrg = c->upload(h.rtrg);
         for (int b = 0; b < BUF_COUNT; ++b) {
             d.bt.stride[b] = h.bufsize[b];

@@ -29,7 +29,9 @@ struct DevPlan {
     int64_t *colB = nullptr, *colC = nullptr, *colA = nullptr;
     FftSrc* fsrc = nullptr;
     VTarget* vtrg = nullptr;
-    VPair* vpair = nullptr;
+    VChunk* vchunk = nullptr;
+    int* vsrc = nullptr;
+    int* vpk = nullptr;
     RTarget* rtrg = nullptr;
     BufTable bt{};
     double2* den_io = nullptr;     // [rhs][N] input order (gather source)

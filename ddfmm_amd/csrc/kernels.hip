@@ -9,7 +9,8 @@
 //                  the reference applies as a per-box zgemv (vecmatop.cpp:102-125)
 //   reduce_kernel  fixed-order sum of the per-contribution slots of the HF M2L/L2L
 //                  targets (the reference's zgemv accumulations into one vector)
-//   fft_fwd_kernel / vlist_kernel   FFT-diagonal LF M2L on the (2P)^3 grid
+//   fft_fwd_kernel / vlist_acc_kernel / vlist_inv_kernel
+//                  FFT-diagonal LF M2L on the (2P)^3 grid
 //                  (wave3d_eval.cpp:602-661)
 //   kgen_kernel    setup-time translation-matrix generation (mlib3d.cpp:106-280 and the
 //                  per-offset HF M2L matrices of wave3d_eval.cpp:876-928)
@@ -574,65 +575,125 @@ __global__ __launch_bounds__(256) void fft_fwd_kernel(const FftSrc* __restrict__
     }
 }
 
+// V list, accumulation: T_t[f] = sum_{(t, s) in V} Tab[offidx(t, s)][f] * S_s[f] over the
+// (2P)^3 frequencies.  A block owns one slice of VSlice<V>::F frequencies (64 B of one
+// spectrum) of one chunk of Morton-consecutive targets.  It stages in LDS the slice of the
+// level's translation tensor (343 offsets) and the slice of every source in the chunk's
+// union (at most VCH_SRC), with independent loads only; the pair loop then reads both
+// operands from LDS.  Items are numbered slice-major and given to the XCDs in contiguous
+// ranges (xcd_item), so an XCD works through a few slices at a time and the source
+// slices it stages come from its own L2.  Threads: F frequencies x 256 / F targets.
+template <class V> struct VSlice {
+    static constexpr int F = 64 / (int)sizeof(V);          // frequencies per slice
+    static constexpr int SE = F + (sizeof(V) == 8 ? 1 : 0); // LDS row stride (FP32 rows padded
+                                                            // to 72 B: fewer bank conflicts)
+};
+
 template <class V>
-__global__ __launch_bounds__(256) void vlist_kernel(const VTarget* __restrict__ vt, const VPair* __restrict__ vp,
-                                                    const int* __restrict__ grid, const V* __restrict__ opm,
-                                                    int NL, BufTable bt, int v0, int nv) {
+__global__ __launch_bounds__(256) void vlist_acc_kernel(const VTarget* __restrict__ vt, const VChunk* __restrict__ vc,
+                                                        const int* __restrict__ vsrc, const int* __restrict__ vpk,
+                                                        const V* __restrict__ opm, BufTable bt, int v0, int c0,
+                                                        int nchunk) {
+    constexpr int G3 = 1728, F = VSlice<V>::F, SE = VSlice<V>::SE, TPP = 256 / F;
+    __shared__ V tab[343 * SE], src[VCH_SRC * SE];
+    const int rhs = blockIdx.y;
+    const int t = threadIdx.x, fl = t % F;
+    const int item = xcd_item(blockIdx.x, (G3 / F) * nchunk);
+    const int f0 = (item / nchunk) * F;
+    const VChunk ch = vc[c0 + item % nchunk];
+    const V* T = opm + vt[v0].tab_off + f0;
+    const V* spec = bufp<V>(bt, BUF_SPEC) + (int64_t)rhs * bt.stride[BUF_SPEC] + f0;
+    {   // staging: every thread's loads are issued before any is used (two round trips);
+        // row 0 of the source rows is zero
+        constexpr int NT = (343 * F + 255) / 256, NS = (VCH_SRC * F + 255) / 256;
+        const int nu = (ch.u1 - ch.u0) * F;
+        V a[NT], b[NS];
+        int si[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int e = t + 256 * k;
+            si[k] = e < nu ? vsrc[ch.u0 + e / F] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+            const int e = t + 256 * k;
+            a[k] = e < 343 * F ? T[(int64_t)(e / F) * G3 + e % F] : mk<V>(0, 0);
+        }
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int e = t + 256 * k;
+            b[k] = e >= F && e < nu ? spec[(int64_t)si[k] * G3 + e % F] : mk<V>(0, 0);
+        }
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+            const int e = t + 256 * k;
+            if (e < 343 * F) tab[(e / F) * SE + e % F] = a[k];
+        }
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int e = t + 256 * k;
+            if (e < nu) src[(e / F) * SE + e % F] = b[k];
+        }
+    }
+    __syncthreads();
+    const V* srcl = src + fl;
+    const V* tabl = tab + fl;
+    V* out = bufp<V>(bt, BUF_TSPEC) + (int64_t)rhs * bt.stride[BUF_TSPEC] + f0 + fl;
+    for (int i = ch.t0 + t / F; i < ch.t1; i += TPP) {
+        const VTarget tg = vt[i];
+        V acc = mk<V>(0, 0);
+        // the target's packed pairs, padded with zero pairs (zero source row) to a multiple
+        // of 8, read F at a time by every lane of its group (same address: a broadcast),
+        // the next F issued before this step's LDS reads
+        const int n = (tg.p1 - tg.p0 + 7) & ~7;
+        const int4* pk = (const int4*)(vpk + tg.k0);
+        auto step = [&](const int4* q4) {
+            V sv[F], tv[F];
+#pragma unroll
+            for (int u = 0; u < F; ++u) {
+                const int4 q = q4[u / 4];
+                const int w = (u % 4 == 0) ? q.x : (u % 4 == 1) ? q.y : (u % 4 == 2) ? q.z : q.w;
+                sv[u] = srcl[(w & 0xffff) * SE];
+                tv[u] = tabl[(w >> 16) * SE];
+            }
+#pragma unroll
+            for (int u = 0; u < F; ++u) cfma(acc, sv[u], tv[u]);
+        };
+        int4 qa[F / 4], qb[F / 4];
+#pragma unroll
+        for (int c = 0; c < F / 4; ++c) qa[c] = pk[c];
+        for (int j = 0; j < n; j += 2 * F) {     // two steps per pass, the next F pairs in flight
+            if (j + F < n) {
+#pragma unroll
+                for (int c = 0; c < F / 4; ++c) qb[c] = pk[(j + F) / 4 + c];
+            }
+            step(qa);
+            if (j + F >= n) break;
+            if (j + 2 * F < n) {
+#pragma unroll
+                for (int c = 0; c < F / 4; ++c) qa[c] = pk[(j + 2 * F) / 4 + c];
+            }
+            step(qb);
+        }
+        out[(int64_t)(i - v0) * G3] = acc;
+    }
+}
+
+// V list, inverse: one block per target, the accumulated spectrum back to the 6^3 sub-cube
+// (pruned inverse DFT, 1/(2P)^3 scaling) and added into the target's check values.
+template <class V>
+__global__ __launch_bounds__(256) void vlist_inv_kernel(const VTarget* __restrict__ vt, const int* __restrict__ grid,
+                                                        int NL, BufTable bt, int v0, int nv) {
     constexpr int G3 = 1728;
-    constexpr int PER = (G3 + 255) / 256;
     __shared__ V g0[G3], A[864], B[432];
     const int rhs = blockIdx.y;
     const int t = threadIdx.x;
-    const V* spec = bufp<V>(bt, BUF_SPEC) + (int64_t)rhs * bt.stride[BUF_SPEC];
     for (int bi = blockIdx.x; bi < nv; bi += gridDim.x) {
-    const VTarget tg = vt[v0 + xcd_item(bi, nv)];
+    const int i = xcd_item(bi, nv);
+    const VTarget tg = vt[v0 + i];
+    const V* in = bufp<V>(bt, BUF_TSPEC) + (int64_t)rhs * bt.stride[BUF_TSPEC] + (int64_t)i * G3;
     __syncthreads();
-    V acc[PER];
-#pragma unroll
-    for (int l = 0; l < PER; ++l) acc[l] = mk<V>(0, 0);
-    // Hadamard products, two source pairs per step (both pairs' loads in flight together)
-    int p = tg.p0;
-    for (; p + 1 < tg.p1; p += 2) {
-        const VPair pa = vp[p], pb = vp[p + 1];
-        const V* sa = spec + pa.spec_off;
-        const V* sb = spec + pb.spec_off;
-        const V* Ta = opm + tg.tab_off + (int64_t)pa.offidx * G3;
-        const V* Tb = opm + tg.tab_off + (int64_t)pb.offidx * G3;
-        V xa[PER], ya[PER], xb[PER], yb[PER];
-#pragma unroll
-        for (int l = 0; l < PER; ++l) {
-            const int e = t + 256 * l;
-            if (e < G3) {
-                xa[l] = sa[e];
-                ya[l] = Ta[e];
-                xb[l] = sb[e];
-                yb[l] = Tb[e];
-            }
-        }
-#pragma unroll
-        for (int l = 0; l < PER; ++l) {
-            const int e = t + 256 * l;
-            if (e < G3) {
-                cfma(acc[l], xa[l], ya[l]);
-                cfma(acc[l], xb[l], yb[l]);
-            }
-        }
-    }
-    if (p < tg.p1) {
-        const VPair pa = vp[p];
-        const V* sa = spec + pa.spec_off;
-        const V* Ta = opm + tg.tab_off + (int64_t)pa.offidx * G3;
-#pragma unroll
-        for (int l = 0; l < PER; ++l) {
-            const int e = t + 256 * l;
-            if (e < G3) cfma(acc[l], sa[e], Ta[e]);
-        }
-    }
-#pragma unroll
-    for (int l = 0; l < PER; ++l) {
-        const int e = t + 256 * l;
-        if (e < G3) g0[e] = acc[l];
-    }
+    for (int e = t; e < G3; e += 256) g0[e] = in[e];
     __syncthreads();
     if (t < 144)         // x lines (ky, kz): 12 inputs -> x' in [6, 12), A[kz][ky][x']
         dft12_line<V, 12, 6, 0, 6, 1>(g0 + 12 * t, 1, A + 6 * t, 1);
@@ -756,20 +817,37 @@ __global__ __launch_bounds__(256) void reduce_kernel(const RTarget* __restrict__
     const int rhs = blockIdx.y;
     const V* in = bufp<V>(bt, BUF_RED) + (int64_t)rhs * bt.stride[BUF_RED] + r.red_off;
     V* out = bufp<V>(bt, r.obuf) + (int64_t)rhs * bt.stride[r.obuf] + r.out_off;
-    for (int m = lane; m < r.M; m += 64) {
-        T sx = 0, sy = 0;
+    // two rows per lane (128 per pass) so that a wave keeps twice the loads in flight;
+    // lanes past M read row M - 1 (in bounds) and store nothing
+    for (int m0 = 0; m0 < r.M; m0 += 128) {
+        const int ma = m0 + lane, mb = m0 + 64 + lane;
+        const int ia = min(ma, r.M - 1), ib = min(mb, r.M - 1);
+        T ax = 0, ay = 0, bx = 0, by = 0;
 #pragma unroll 4
         for (int j = 0; j < r.cnt; ++j) {
-            const V v = in[(int64_t)j * r.M + m];
-            sx += v.x;
-            sy += v.y;
+            const V* s = in + (int64_t)j * r.M;
+            const V va = s[ia], vb = s[ib];
+            ax += va.x;
+            ay += va.y;
+            bx += vb.x;
+            by += vb.y;
         }
-        if (r.mode) {
-            const V o = out[m];
-            sx += o.x;
-            sy += o.y;
+        if (ma < r.M) {
+            if (r.mode) {
+                const V o = out[ma];
+                ax += o.x;
+                ay += o.y;
+            }
+            out[ma] = mk<V>(ax, ay);
         }
-        out[m] = mk<V>(sx, sy);
+        if (mb < r.M) {
+            if (r.mode) {
+                const V o = out[mb];
+                bx += o.x;
+                by += o.y;
+            }
+            out[mb] = mk<V>(bx, by);
+        }
     }
 }
 
@@ -824,15 +902,18 @@ hipError_t launch_stage_t(const DevPlan& d, const Stage& s, int nrhs, hipStream_
         fft_fwd_kernel<V><<<dim3(s.stream ? std::min(n, AUX_BLOCKS) : n, nrhs), 256, 0, st>>>(d.fsrc, d.grid, d.NL,
                                                                                               d.bt, s.begin, n);
         break;
-    case ST_VLIST:
+    case ST_VLIST: {
         if (d.P != 6) return hipErrorInvalidValue;
-        if (d.mixed)
-            vlist_kernel<float2><<<dim3(s.stream ? std::min(n, AUX_BLOCKS) : n, nrhs), 256, 0, st>>>(
-                d.vtrg, d.vpair, d.grid, d.opmat32, d.NL, d.bt, s.begin, n);
-        else
-            vlist_kernel<double2><<<dim3(s.stream ? std::min(n, AUX_BLOCKS) : n, nrhs), 256, 0, st>>>(
-                d.vtrg, d.vpair, d.grid, d.opmat, d.NL, d.bt, s.begin, n);
+        if (n == 0) break;
+        constexpr int F = VSlice<V>::F;
+        const int nchunk = (int)s.count;
+        vlist_acc_kernel<V><<<dim3((1728 / F) * nchunk, nrhs), 256, 0, st>>>(
+            d.vtrg, d.vchunk, d.vsrc, d.vpk, d.mixed ? (const V*)d.opmat32 : (const V*)d.opmat, d.bt, s.begin, s.buf,
+            nchunk);
+        vlist_inv_kernel<V><<<dim3(s.stream ? std::min(n, AUX_BLOCKS) : n, nrhs), 256, 0, st>>>(d.vtrg, d.grid, d.NL,
+                                                                                                d.bt, s.begin, n);
         break;
+    }
     case ST_REDUCE:
         reduce_kernel<V><<<dim3((n + 3) / 4, nrhs), 256, 0, st>>>(d.rtrg, n, d.bt, s.begin);
         break;

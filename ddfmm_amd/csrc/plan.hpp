@@ -22,6 +22,7 @@ enum Buf : int {
     BUF_HF_B,
     BUF_SPEC,       // LF V-list source spectra of one level, (2P)^3 each
     BUF_RED,        // HF M2L/L2L contributions of one level, one slot per (target, source)
+    BUF_TSPEC,      // LF V-list target spectra of one level, (2P)^3 each, in V-target order
     BUF_COUNT
 };
 // setup-time GEMMs read and write the operator table itself (FP64), as buffer BUF_OPMAT
@@ -80,18 +81,32 @@ struct FftSrc {
     int grid;           // grid-index table offset
     int pad;
 };
+// One ST_VLIST stage covers the targets of one level: their spectra accumulate, per chunk
+// and frequency slice, into BUF_TSPEC (target i of the stage at i * (2P)^3), then one
+// inverse transform per target adds them into LF_CHK.
 struct VTarget {
     int64_t out_off;    // LF_CHK slot
     int p0, p1;         // pair range
-    int64_t tab_off;    // translation tensor table of this width (343 x (2P)^3)
+    int64_t tab_off;    // translation tensor table of this width (343 x (2P)^3), same for the stage
     int grid;
-    int pad;
+    int k0;             // first packed pair in vpk (a multiple of 8; the target's p1 - p0 packed
+                        // pairs, then zero pairs up to a multiple of 8)
 };
 struct VPair {
     int64_t spec_off;
-    int offidx;         // (a+3) + 7(b+3) + 49(c+3)
+    int offidx;         // (a+3) + 7(b+3) + 49(c+3), (a, b, c) = target - source in box widths
     int pad;
 };
+// V-list chunk: Morton-consecutive targets [t0, t1) of one level whose V-list sources, the
+// union vsrc[u0, u1) of at most VCH_SRC rows (row 0 a zero row, then source spectrum
+// indices), are staged in LDS together.  The device reads each pair packed as
+// local row | (offidx << 16); a zero entry (row 0) pads.
+struct VChunk {
+    int t0, t1;
+    int u0, u1;
+};
+constexpr int VCH_SRC = 400;
+constexpr int VCH_TRG = 256;
 
 // segmented reduction: out[out_off + m] (=|+=) sum_{j < cnt} BUF_RED[red_off + j * M + m]
 // (the contributions of one target are adjacent, in a fixed order: deterministic sums)
@@ -128,8 +143,8 @@ enum StagePrec : int { PREC_TRANSLATION = 0, PREC_SOLVE = 1 };
 struct Stage {
     int kind;
     int begin, end;     // descriptor range (tasks / tiles / sources / targets)
-    int buf;            // ST_ZERO: buffer id; ST_FORK/ST_JOIN: event index
-    int64_t count;      // ST_ZERO: elements per rhs
+    int buf;            // ST_ZERO: buffer id; ST_FORK/ST_JOIN: event index; ST_VLIST: first chunk
+    int64_t count;      // ST_ZERO: elements per rhs; ST_VLIST: chunks
     int prec;           // StagePrec
     int stream;         // 0: the evaluation (caller's) stream, 1: the concurrent second stream
     double work;        // per RHS: GEMM useful flops, KAPPLY interactions, FFT spectra, V-list pairs,

@@ -251,7 +251,41 @@ struct Builder {
             }
             if (st.kind == ST_KAPPLY) need(st.end <= (int64_t)pl.ktasks.size(), "kapply stage");
             if (st.kind == ST_FFTFWD) need(st.end <= (int64_t)pl.fsrc.size(), "fft stage");
-            if (st.kind == ST_VLIST) need(st.end <= (int64_t)pl.vtrg.size(), "vlist stage");
+            if (st.kind == ST_VLIST) {
+                need(st.end <= (int64_t)pl.vtrg.size(), "vlist stage");
+                need((int64_t)(st.end - st.begin) * G3 <= pl.bufsize[BUF_TSPEC], "vlist target spectra");
+                for (int t = st.begin; t < st.end; ++t)
+                    need(pl.vtrg[t].tab_off == pl.vtrg[st.begin].tab_off, "vlist stage table");
+                // chunks tile the stage's targets in order; every pair's local index names
+                // its own source in the chunk's union
+                need(st.buf >= 0 && st.count >= 1 && st.buf + st.count <= (int64_t)pl.vchunk.size(), "vlist chunks");
+                int next = st.begin;
+                for (int64_t c = st.buf; c < st.buf + st.count; ++c) {
+                    const VChunk& ch = pl.vchunk[c];
+                    need(ch.t0 == next && ch.t0 < ch.t1 && ch.t1 <= st.end, "vlist chunk targets");
+                    need(ch.u0 >= 0 && ch.u0 <= ch.u1 && ch.u1 - ch.u0 <= VCH_SRC &&
+                             ch.u1 <= (int)pl.vsrc.size(), "vlist chunk sources");
+                    for (int u = ch.u0; u < ch.u1; ++u)
+                        need(pl.vsrc[u] >= 0 && (int64_t)pl.vsrc[u] * G3 + G3 <= pl.bufsize[BUF_SPEC], "vlist chunk source");
+                    for (int t = ch.t0; t < ch.t1; ++t) {
+                        const VTarget& v = pl.vtrg[t];
+                        const int n8 = (v.p1 - v.p0 + 7) & ~7;
+                        need(v.k0 >= 0 && v.k0 % 8 == 0 && v.k0 + n8 <= (int)pl.vpk.size(), "vlist packed range");
+                        for (int j = 0; j < n8; ++j) {
+                            const int w = pl.vpk[v.k0 + j], l = w & 0xffff, o = w >> 16;
+                            if (j >= v.p1 - v.p0) {
+                                need(w == 0, "vlist packed padding");
+                                continue;
+                            }
+                            const VPair& pr = pl.vpair[v.p0 + j];
+                            need(l >= 1 && l < ch.u1 - ch.u0 && (int64_t)pl.vsrc[ch.u0 + l] * G3 == pr.spec_off &&
+                                     o == pr.offidx, "vlist packed pair");
+                        }
+                    }
+                    next = ch.t1;
+                }
+                need(next == st.end, "vlist chunks cover the stage");
+            }
             if (st.kind == ST_REDUCE) need(st.end <= (int64_t)pl.rtrg.size(), "reduce stage");
             if (st.kind == ST_ZERO) need(st.buf >= 0 && st.buf < BUF_COUNT && st.count <= pl.bufsize[st.buf], "zero");
             if (st.kind == ST_FORK || st.kind == ST_JOIN) need(st.buf >= 0 && st.buf < STAGE_EVENTS, "sync event");
@@ -1005,7 +1039,7 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
     }
 
     // ---- LF downward (W descending), EvalDownwardLow 493-583
-    int64_t spec_max = 0;
+    int64_t spec_max = 0, vtrg_max = 0;
     for (auto lt = ldmap.rbegin(); lt != ldmap.rend(); ++lt) {
         double W = lt->first;
         LfW& w = lfw[W];
@@ -1050,8 +1084,65 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
             pl.stats.v_pairs += (int64_t)b.V.size();
         }
         pl.stats.v_targets += (int64_t)pl.vtrg.size() - v0;
+        vtrg_max = std::max(vtrg_max, (int64_t)pl.vtrg.size() - v0);
         pl.stats.v_sources += (int64_t)specidx.size();
-        stage(ST_VLIST, v0, (int)pl.vtrg.size(), "lf_vlist");
+        // chunks of Morton-consecutive targets, closed when the union of their sources
+        // would exceed VCH_SRC rows or the chunk holds VCH_TRG targets; row 0 of every
+        // union is a zero row (the kernel stages zeros there) that pads partial pair batches
+        const int c0 = (int)pl.vchunk.size();
+        std::vector<int> lpk(pl.vpair.size());      // packed pair q (chunk-local row, offset)
+        {
+            std::unordered_map<int64_t, int> loc;
+            VChunk ch{};
+            auto open = [&](int t0) {
+                ch.t0 = t0;
+                ch.u0 = (int)pl.vsrc.size();
+                pl.vsrc.push_back(0);
+                loc.clear();
+            };
+            auto close = [&](int t1) {
+                ch.t1 = t1;
+                ch.u1 = (int)pl.vsrc.size();
+                pl.vchunk.push_back(ch);
+            };
+            open(v0);
+            for (int i = v0; i < (int)pl.vtrg.size(); ++i) {
+                const VTarget& v = pl.vtrg[i];
+                int add = 0;
+                for (int q = v.p0; q < v.p1; ++q) add += !loc.count(pl.vpair[q].spec_off / G3);
+                if (i > ch.t0 && (1 + (int)loc.size() + add > VCH_SRC || i - ch.t0 >= VCH_TRG)) {
+                    close(i);
+                    open(i);
+                }
+                for (int q = v.p0; q < v.p1; ++q) {
+                    const int64_t s = pl.vpair[q].spec_off / G3;
+                    auto it = loc.find(s);
+                    int l;
+                    if (it == loc.end()) {
+                        l = 1 + (int)loc.size();
+                        loc.emplace(s, l);
+                        pl.vsrc.push_back((int)s);
+                    } else {
+                        l = it->second;
+                    }
+                    lpk[q] = l | (pl.vpair[q].offidx << 16);
+                }
+            }
+            if ((int)pl.vtrg.size() > ch.t0) close((int)pl.vtrg.size());
+            else pl.vsrc.pop_back();
+        }
+        // within a chunk, targets by descending pair count: the targets that share a wave
+        // run loops of about the same length (a target's slot follows its new position)
+        for (int c = c0; c < (int)pl.vchunk.size(); ++c)
+            std::stable_sort(pl.vtrg.begin() + pl.vchunk[c].t0, pl.vtrg.begin() + pl.vchunk[c].t1,
+                             [](const VTarget& a, const VTarget& b) { return a.p1 - a.p0 > b.p1 - b.p0; });
+        for (int i = v0; i < (int)pl.vtrg.size(); ++i) {
+            VTarget& v = pl.vtrg[i];
+            v.k0 = (int)pl.vpk.size();
+            for (int q = v.p0; q < v.p1; ++q) pl.vpk.push_back(lpk[q]);
+            while (pl.vpk.size() % 8) pl.vpk.push_back(0);
+        }
+        stage(ST_VLIST, v0, (int)pl.vtrg.size(), "lf_vlist", c0, (int64_t)pl.vchunk.size() - c0);
         // X list into the check potential
         int t0 = (int)pl.ktasks.size();
         for (auto& k : lt->second) {
@@ -1117,6 +1208,7 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
         }
     }
     pl.bufsize[BUF_SPEC] = spec_max * G3;
+    pl.bufsize[BUF_TSPEC] = vtrg_max * G3;
 
     // ---- leaf kernel (eval 550-560, 585-718), in two launches: the near field (U list,
     // W and X lists evaluated directly from point densities) needs only the densities and

@@ -43,6 +43,9 @@ struct HostPlan {
     std::vector<FftSrc> fsrc;
     std::vector<VTarget> vtrg;
     std::vector<VPair> vpair;
+    std::vector<VChunk> vchunk;
+    std::vector<int> vsrc;                  // chunk source unions (spectrum indices)
+    std::vector<int> vpk;                   // packed pairs per target, padded to multiples of 8
     std::vector<RTarget> rtrg;
     std::vector<Stage> stages;
     std::vector<Stage> setup_stages;       // FP64 GEMMs on the operator table, run once after generation

@@ -807,47 +807,60 @@ __global__ void kmatrix_kernel(const double* __restrict__ trg, int M, const doub
 }
 
 // segmented reduction: target t gets the sum of its cnt adjacent contribution slots
-// (M elements each) in a fixed order; one wave per target, four targets per block
+// (M elements each) in a fixed order; one wave per target, four targets per block.  A lane
+// owns rows lane + 64 r (r < R, R = 2 for M <= 128, 3 for M <= 192) and walks the slots
+// once, all R loads of a slot in flight together; lanes past M read row M - 1 (in bounds)
+// and store nothing.
+template <class V, int R>
+__device__ __forceinline__ void reduce_rows(const V* __restrict__ in, V* __restrict__ out, int M, int cnt, int mode,
+                                            int m0, int lane) {
+    using T = typename Real<V>::T;
+    int idx[R];
+    T sx[R], sy[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        idx[r] = min(m0 + lane + 64 * r, M - 1);
+        sx[r] = 0;
+        sy[r] = 0;
+    }
+#pragma unroll 4
+    for (int j = 0; j < cnt; ++j) {
+        const V* s = in + (int64_t)j * M;
+        V v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = s[idx[r]];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            sx[r] += v[r].x;
+            sy[r] += v[r].y;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int m = m0 + lane + 64 * r;
+        if (m < M) {
+            if (mode) {
+                const V o = out[m];
+                sx[r] += o.x;
+                sy[r] += o.y;
+            }
+            out[m] = mk<V>(sx[r], sy[r]);
+        }
+    }
+}
+
 template <class V>
 __global__ __launch_bounds__(256) void reduce_kernel(const RTarget* __restrict__ rtrg, int n, BufTable bt, int t0) {
-    using T = typename Real<V>::T;
     const int ti = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (ti >= n) return;
     const RTarget r = rtrg[t0 + ti];
     const int rhs = blockIdx.y;
     const V* in = bufp<V>(bt, BUF_RED) + (int64_t)rhs * bt.stride[BUF_RED] + r.red_off;
     V* out = bufp<V>(bt, r.obuf) + (int64_t)rhs * bt.stride[r.obuf] + r.out_off;
-    // two rows per lane (128 per pass) so that a wave keeps twice the loads in flight;
-    // lanes past M read row M - 1 (in bounds) and store nothing
-    for (int m0 = 0; m0 < r.M; m0 += 128) {
-        const int ma = m0 + lane, mb = m0 + 64 + lane;
-        const int ia = min(ma, r.M - 1), ib = min(mb, r.M - 1);
-        T ax = 0, ay = 0, bx = 0, by = 0;
-#pragma unroll 4
-        for (int j = 0; j < r.cnt; ++j) {
-            const V* s = in + (int64_t)j * r.M;
-            const V va = s[ia], vb = s[ib];
-            ax += va.x;
-            ay += va.y;
-            bx += vb.x;
-            by += vb.y;
-        }
-        if (ma < r.M) {
-            if (r.mode) {
-                const V o = out[ma];
-                ax += o.x;
-                ay += o.y;
-            }
-            out[ma] = mk<V>(ax, ay);
-        }
-        if (mb < r.M) {
-            if (r.mode) {
-                const V o = out[mb];
-                bx += o.x;
-                by += o.y;
-            }
-            out[mb] = mk<V>(bx, by);
-        }
+    if (r.M <= 128) {
+        reduce_rows<V, 2>(in, out, r.M, r.cnt, r.mode, 0, lane);
+    } else {
+        for (int m0 = 0; m0 < r.M; m0 += 192) reduce_rows<V, 3>(in, out, r.M, r.cnt, r.mode, m0, lane);
     }
 }
 

@@ -58,6 +58,86 @@ CMat matmul(const CMat& A, const CMat& B) {
     return C;
 }
 
+// Column order for a K-blocked GEMM whose columns skip absent K blocks (child masks): a
+// wave of 16 columns multiplies the union of its columns' blocks, a tile of four waves
+// loads the union of theirs.  Columns of one mask fill whole waves first (masks in
+// Gray-code order); the remainders are packed greedily, the next column always the first
+// one (by popcount, then Gray code) that adds the fewest new blocks to the wave; waves are
+// then grouped into tiles the same way.  Deterministic for a given input order.
+template <class K>
+void pack_by_mask(std::vector<std::pair<int, K>>& cols) {
+    auto gray = [](int m) { return m ^ (m >> 1); };
+    std::map<int, std::vector<size_t>> by;     // gray code -> column indices in input order
+    for (size_t i = 0; i < cols.size(); ++i) by[gray(cols[i].first)].push_back(i);
+    std::vector<size_t> order, left;
+    order.reserve(cols.size());
+    for (auto& kv : by) {
+        const size_t full = kv.second.size() / 16 * 16;
+        order.insert(order.end(), kv.second.begin(), kv.second.begin() + full);
+        left.insert(left.end(), kv.second.begin() + full, kv.second.end());
+    }
+    std::stable_sort(left.begin(), left.end(), [&](size_t a, size_t b) {
+        return __builtin_popcount(cols[a].first) > __builtin_popcount(cols[b].first);
+    });
+    std::vector<char> used(left.size(), 0);
+    for (size_t i = 0; i < left.size(); ++i) {
+        if (used[i]) continue;
+        used[i] = 1;
+        order.push_back(left[i]);
+        int u = cols[left[i]].first;
+        for (int n = 1; n < 16; ++n) {
+            int best = -1, bg = 99;
+            for (size_t j = i + 1; j < left.size() && bg > 0; ++j) {
+                if (used[j]) continue;
+                const int g = __builtin_popcount(u | cols[left[j]].first) - __builtin_popcount(u);
+                if (g < bg) { bg = g; best = (int)j; }
+            }
+            if (best < 0) break;
+            used[best] = 1;
+            order.push_back(left[best]);
+            u |= cols[left[best]].first;
+        }
+    }
+    // waves of 16 into tiles of 4 (GEMM_TN columns): a tile loops over the union of its
+    // waves' blocks, so each tile takes the first free wave and the three that add the
+    // fewest blocks to it; a partial wave (the last one) stays last
+    std::vector<std::pair<size_t, int>> waves;      // (first position in order, union mask)
+    for (size_t s = 0; s < order.size(); s += 16) {
+        int u = 0;
+        for (size_t j = s; j < std::min(order.size(), s + 16); ++j) u |= cols[order[j]].first;
+        waves.push_back({s, u});
+    }
+    const bool partial = order.size() % 16 != 0;
+    const size_t nfull = waves.size() - (partial ? 1 : 0);
+    std::vector<size_t> wseq;
+    std::vector<char> wused(nfull, 0);
+    for (size_t i = 0; i < nfull; ++i) {
+        if (wused[i]) continue;
+        wused[i] = 1;
+        wseq.push_back(i);
+        int u = waves[i].second;
+        for (int n = 1; n < GEMM_TN / 16; ++n) {
+            int best = -1, bg = 99;
+            for (size_t j = i + 1; j < nfull && bg > 0; ++j) {
+                if (wused[j]) continue;
+                const int g = __builtin_popcount(u | waves[j].second) - __builtin_popcount(u);
+                if (g < bg) { bg = g; best = (int)j; }
+            }
+            if (best < 0) break;
+            wused[best] = 1;
+            wseq.push_back(best);
+            u |= waves[best].second;
+        }
+    }
+    if (partial) wseq.push_back(waves.size() - 1);
+    std::vector<std::pair<int, K>> out;
+    out.reserve(cols.size());
+    for (size_t w : wseq)
+        for (size_t j = waves[w].first; j < std::min(order.size(), waves[w].first + 16); ++j)
+            out.push_back(cols[order[j]]);
+    cols.swap(out);
+}
+
 // which independent sections run on the second stream
 constexpr bool SECOND_STREAM_NEAR = true;   // near field (U list, direct W/X lists)
 constexpr bool SECOND_STREAM_LF = true;     // FFT, V list, X list per LF width
@@ -834,9 +914,8 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
         int g0 = (int)pl.tiles.size();
         if (w.has_children) {
             // one GEMM: [ue2uc_0 .. ue2uc_7] (NL x 8 NL) times the children stacked along K;
-            // columns sorted by child-occupancy mask (Gray-code order: neighbouring masks
-            // differ in one child) so that a wave's union of present children (the K blocks
-            // it multiplies) stays small
+            // columns ordered by child-occupancy mask (pack_by_mask) so that a wave's union
+            // of present children (the K blocks it multiplies) stays small
             std::vector<std::pair<int, BoxKey>> cols;
             for (auto& k : kv.second) {
                 if (boxes.at(k).terminal()) continue;
@@ -845,10 +924,7 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
                     if (lfidx.count(Tree::child_key(k, c))) mask |= 1 << c;
                 cols.push_back({mask, k});
             }
-            std::stable_sort(cols.begin(), cols.end(),
-                             [](const std::pair<int, BoxKey>& a, const std::pair<int, BoxKey>& b) {
-                                 return (a.first ^ (a.first >> 1)) < (b.first ^ (b.first >> 1));
-                             });
+            pack_by_mask(cols);
             std::vector<int64_t> bb, cc, ab;
             for (auto& mk : cols) {
                 for (int c = 0; c < 8; ++c) {
@@ -895,7 +971,7 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
         for (auto& h : lv.second) {
             Index3 pd = W == 1.0 ? Index3() : parent_dir(h.dir);
             // one GEMM per direction: [ue2uc_0 .. ue2uc_7] (r2 x 8 r1c), children stacked
-            // along K, columns sorted by child-occupancy mask (see the LF M2M above)
+            // along K, columns ordered by child-occupancy mask (pack_by_mask)
             std::vector<std::pair<int, BoxKey>> cols;
             for (auto& k : h.out) {
                 int mask = 0;
@@ -905,10 +981,7 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
                 }
                 cols.push_back({mask, k});
             }
-            std::stable_sort(cols.begin(), cols.end(),
-                             [](const std::pair<int, BoxKey>& a, const std::pair<int, BoxKey>& b) {
-                                 return (a.first ^ (a.first >> 1)) < (b.first ^ (b.first >> 1));
-                             });
+            pack_by_mask(cols);
             std::vector<int64_t> bb, cc, ab;
             for (auto& mk : cols) {
                 for (int c = 0; c < 8; ++c) {

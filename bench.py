@@ -111,10 +111,10 @@ def main():
                     help="one 256 MiB device-to-device copy (blit kernel) to calibrate PMC byte counters")
     ap.add_argument("--chk", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--serve", default="sphere8_60k,sphere16_250k_x64",
+    ap.add_argument("--serve", default="sphere8_60k,sphere16_250k_x64,sphere32_1m,ellipsoid_4m_x8",
                     help="comma-separated geometries for the serving leg ('' to skip)")
-    ap.add_argument("--serve-qps", type=float, default=40.0)
-    ap.add_argument("--serve-requests", type=int, default=80)
+    ap.add_argument("--serve-qps", type=float, default=10.0)
+    ap.add_argument("--serve-requests", type=int, default=48)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "

@@ -740,9 +740,42 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
         auto bi = boxes.find(ck);
         return bi != boxes.end() && bi->second.has_pts();
     };
+    // Width 1 -> LF children: the incoming directions of one class (equal solve size, so
+    // equal K blocks) are bundled by up to 8 into one K-blocked GEMM per (bundle, child
+    // position) whose columns are the boxes with any of the bundle's directions: one LF
+    // check contribution per bundle instead of one per direction.  Columns: the union of
+    // the bundle's in-lists, each with its direction-presence mask.
+    struct Bundle {
+        std::vector<const HfDir*> d;
+        std::map<BoxKey, int> cols;
+    };
+    std::vector<Bundle> w1b;
+    if (hfl.count(1.0)) {
+        std::map<int, std::vector<const HfDir*>> cls;
+        for (auto& h : hfl.at(1.0)) cls[prm.fuse_solves ? h.r1 : h.r2].push_back(&h);
+        for (auto& kv : cls)
+            for (size_t s = 0; s < kv.second.size(); s += 8) {
+                Bundle b;
+                for (size_t j = s; j < std::min(kv.second.size(), s + 8); ++j) {
+                    const int bit = 1 << (int)b.d.size();
+                    b.d.push_back(kv.second[j]);
+                    for (auto& k : kv.second[j]->in) b.cols[k] |= bit;
+                }
+                w1b.push_back(std::move(b));
+            }
+    }
     int lf_r0 = -1;
     for (auto lt = hfl.rbegin(); lt != hfl.rend(); ++lt) {
-        if (lt->first == 1.0) lf_r0 = (int)rts.size();
+        if (lt->first == 1.0) {
+            lf_r0 = (int)rts.size();
+            for (auto& b : w1b)
+                for (int c = 0; c < 8; ++c)
+                    for (auto& km : b.cols) {
+                        BoxKey ck = Tree::child_key(km.first, c);
+                        if (child_present(ck)) rts[l2l_target(*b.d[0], ck)].cnt++;
+                    }
+            continue;
+        }
         for (auto& h : lt->second)
             for (int c = 0; c < 8; ++c)
                 for (auto& k : h.in) {
@@ -1069,7 +1102,36 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
             }
         }
         g0 = (int)pl.tiles.size();
+        if (W == 1.0) {
+            // bundled: C(LF child c) = sum_b A_b(c) x(box, dir_b), A_b(c) = D_c (fused) or
+            // de2dc_c (factored) of direction b, K blocks of the bundle's solve size
+            for (auto& b : w1b) {
+                const int kb = prm.fuse_solves ? b.d[0]->r1 : b.d[0]->r2;
+                for (int c = 0; c < 8; ++c) {
+                    std::vector<std::pair<int, BoxKey>> cols;
+                    for (auto& km : b.cols)
+                        if (child_present(Tree::child_key(km.first, c))) cols.push_back({km.second, km.first});
+                    if (cols.empty()) continue;
+                    pack_by_mask(cols);
+                    std::vector<int64_t> bb, cc, ab;
+                    for (auto& mk : cols) {
+                        for (size_t j = 0; j < b.d.size(); ++j) {
+                            if (!(mk.first & (1 << j))) { bb.push_back(-1); continue; }
+                            bb.push_back(prm.fuse_solves ? inslot.at({mk.second, b.d[j]->dir})
+                                                         : iscr.at({mk.second, b.d[j]->dir}));
+                        }
+                        cc.push_back(slot(l2l_target(*b.d[0], Tree::child_key(mk.second, c))));
+                    }
+                    for (auto* h : b.d) ab.push_back(prm.fuse_solves ? G_(h->D[c]) : G_(h->de2dc[c]));
+                    GGroup g{};
+                    g.aoff = -1; g.soff = -1; g.M = b.d[0]->r1c; g.K = (int)b.d.size() * kb; g.lda = b.d[0]->r1c;
+                    g.transA = 0; g.bbuf = prm.fuse_solves ? BUF_HF_DN : BUF_HF_A; g.cbuf = BUF_RED; g.mode = 0;
+                    add_group(g, bb, cc, nullptr, kb, ab);
+                }
+            }
+        }
         for (auto& h : lt->second) {
+            if (W == 1.0) break;
             for (int c = 0; c < 8; ++c) {
                 std::vector<int64_t> bc, cc;
                 for (auto& k : h.in) {

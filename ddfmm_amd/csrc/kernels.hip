@@ -97,10 +97,10 @@ __device__ __forceinline__ int xcd_item(int b, int n) {
 }
 
 // ---------------------------------------------------------------- kernel apply
-constexpr int KA_THREADS = 128;
+constexpr int KA_THREADS = 256;
 
-// One block per task.  A pass covers nt = min(count, 128) targets; when nt < 128 the
-// 128 threads split into G = 128 / nt source groups (thread = target x group) that
+// One block per task.  A pass covers nt = min(count, 256) targets; when nt < 256 the
+// 256 threads split into G = 256 / nt source groups (thread = target x group) that
 // stride through each LDS source tile, and the partial sums are reduced through LDS.
 template <class V, int RC>
 __global__ __launch_bounds__(KA_THREADS) void kapply_kernel(

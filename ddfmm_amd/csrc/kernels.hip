@@ -238,7 +238,7 @@ template <class TA, class V, class R, int MB>
 __global__ __launch_bounds__(256) void gemm_kernel(
     const GGroup* __restrict__ groups, const GTile* __restrict__ tiles,
     const int64_t* __restrict__ colB, const int64_t* __restrict__ colC, const int64_t* __restrict__ colA,
-    const TA* __restrict__ opm, BufTable bt, int tile0) {
+    const TA* __restrict__ opm, BufTable bt, int tile0, int nr) {
     using M = Mfma<R>;
     using T = typename Real<V>::T;
     using RC = typename Lds<R>::C;
@@ -246,9 +246,12 @@ __global__ __launch_bounds__(256) void gemm_kernel(
     constexpr int NA = TM * KS / 256, NB = GEMM_TN * KS / 256;   // elements per thread and slab
     const GTile tl = tiles[tile0 + xcd_item(blockIdx.x, gridDim.x)];
     const GGroup g = groups[tl.g];
-    const int rhs = blockIdx.y;
-    const V* B = bufp<V>(bt, g.bbuf) + (int64_t)rhs * bt.stride[g.bbuf];
-    V* C = bufp<V>(bt, g.cbuf) + (int64_t)rhs * bt.stride[g.cbuf];
+    // columns are (box, rhs) pairs, rhs fastest: column n is box n / nr of the group in
+    // right-hand side n % nr (that rhs slab of the buffer)
+    const V* B = bufp<V>(bt, g.bbuf);
+    V* C = bufp<V>(bt, g.cbuf);
+    const int64_t sB = bt.stride[g.bbuf], sC = bt.stride[g.cbuf];
+    const int ncx = g.ncols * nr;
     // f32: two LDS slab buffers (one barrier per slab); f64 slabs are twice the size and
     // take twice the MFMA time per k, so one buffer and two barriers per slab
     constexpr int NBUF = sizeof(R) == 4 ? 2 : 1;
@@ -294,26 +297,28 @@ __global__ __launch_bounds__(256) void gemm_kernel(
     }
     // this lane's output column, fetched early (the epilogue needs it last)
     const int ncol = tl.n0 + wave * 16 + li;
-    const int64_t coff = ncol < g.ncols ? colC[g.col0 + ncol] : -1;
+    const int64_t coff = ncol < ncx ? colC[g.col0 + ncol / nr] + (int64_t)(ncol % nr) * sC : -1;
 
     // segments.  Contiguous B (kb = 0): one segment, column offsets read straight into
     // registers, waves with columns run.  Blocked B: a table of the K blocks present in
     // the tile's columns, with per-wave masks.
     int nact = 1, wm0 = 0;
     if (g.kb == 0) {
-        const int nv = g.ncols - tl.n0;
+        const int nv = ncx - tl.n0;
         wm0 = nv >= GEMM_TN ? 15 : (1 << ((nv + 15) >> 4)) - 1;
 #pragma unroll
         for (int l = 0; l < NB; ++l) {
             const int n = tl.n0 + (tid + 256 * l) / KS;
-            bc[l] = n < g.ncols ? colB[g.bcol0 + n] : -1;
+            bc[l] = n < ncx ? colB[g.bcol0 + n / nr] + (int64_t)(n % nr) * sB : -1;
         }
     } else {
         if (tid < MAXKB * 4) used[tid >> 2][tid & 3] = 0;
         __syncthreads();
         for (int e = tid; e < nkb * GEMM_TN; e += 256) {
             const int h = e / GEMM_TN, j = e % GEMM_TN;
-            const int64_t v = tl.n0 + j < g.ncols ? colB[g.bcol0 + (int64_t)(tl.n0 + j) * nkb + h] : -1;
+            const int n = tl.n0 + j;
+            int64_t v = n < ncx ? colB[g.bcol0 + (int64_t)(n / nr) * nkb + h] : -1;
+            if (v >= 0) v += (int64_t)(n % nr) * sB;
             bo[h][j] = v;
             if (v >= 0) used[h][j >> 4] = 1;
         }
@@ -500,11 +505,11 @@ __global__ __launch_bounds__(256) void gemm_kernel(
 
 template <class TA, class V, class R>
 void launch_gemm(int mb, int n, int nrhs, hipStream_t st, const DevPlan& d, const TA* opm, int t0) {
-    const dim3 grid(n, nrhs);
+    const dim3 grid(n, 1);
 #define DDFMM_GEMM_MB(X)                                                                                \
     case X:                                                                                             \
         gemm_kernel<TA, V, R, X><<<grid, 256, 0, st>>>(d.groups, d.tiles, d.colB, d.colC, d.colA, opm, \
-                                                       d.bt, t0);                                       \
+                                                       d.bt, t0, nrhs);                                 \
         break;
     switch (mb) {
         DDFMM_GEMM_MB(1) DDFMM_GEMM_MB(2) DDFMM_GEMM_MB(3) DDFMM_GEMM_MB(4) DDFMM_GEMM_MB(5)

@@ -50,13 +50,6 @@ struct ddfmm_ctx {
     bool timing = false;
     std::vector<hipEvent_t> ev;
     std::vector<double> stage_ms;
-    // GEMM tiles over (box, rhs) columns for nrhs > 1, built on first use of each nrhs
-    struct RhsTiles {
-        GTile* dev = nullptr;
-        std::vector<GTile> host;
-        std::vector<std::pair<int, int>> ranges;
-    };
-    std::map<int, RhsTiles> rhs_tiles;
 
     template <class T>
     T* alloc(int64_t n) {
@@ -231,18 +224,8 @@ int ddfmm_setup(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_params* 
 // makes the second stream wait for the work issued so far on `st`, ST_JOIN the reverse,
 // so that the caller's stream sees a complete evaluation.  With timing on, each stage is
 // bracketed by events on its own stream.
-static const ddfmm_ctx::RhsTiles* rhs_tiles(ddfmm_ctx* c, int nrhs) {
-    auto it = c->rhs_tiles.find(nrhs);
-    if (it != c->rhs_tiles.end()) return &it->second;
-    ddfmm_ctx::RhsTiles r;
-    expand_gemm_tiles(c->hp, nrhs, r.host, r.ranges);
-    r.dev = c->upload(r.host);
-    return &c->rhs_tiles.emplace(nrhs, std::move(r)).first->second;
-}
-
 static int run_stages(ddfmm_ctx* c, int nrhs, hipStream_t st) {
     const auto& stages = c->hp.stages;
-    const ddfmm_ctx::RhsTiles* rt = nrhs > 1 ? rhs_tiles(c, nrhs) : nullptr;
     if (c->timing && c->ev.size() != 2 * stages.size()) {
         for (auto e : c->ev) (void)hipEventDestroy(e);
         c->ev.assign(2 * stages.size(), nullptr);
@@ -262,17 +245,7 @@ static int run_stages(ddfmm_ctx* c, int nrhs, hipStream_t st) {
         }
         hipStream_t ss = s.stream ? c->aux : st;
         if (c->timing) HIPCHK(hipEventRecord(c->ev[2 * i], ss));
-        hipError_t e;
-        if (rt && s.kind == ST_GEMM) {      // one launch over the (box, rhs) columns
-            Stage q = s;
-            q.begin = rt->ranges[i].first;
-            q.end = rt->ranges[i].second;
-            DevPlan d = c->dp;
-            d.tiles = rt->dev;
-            e = launch_stage(d, q, nrhs, ss);
-        } else {
-            e = launch_stage(c->dp, s, nrhs, ss);
-        }
+        hipError_t e = launch_stage(c->dp, s, nrhs, ss);
         if (e != hipSuccess) throw HipFail(std::string("stage ") + s.name + ": " + hipGetErrorString(e));
         if (c->timing) HIPCHK(hipEventRecord(c->ev[2 * i + 1], ss));
     }

@@ -244,14 +244,19 @@ __global__ __launch_bounds__(256) void gemm_kernel(
     using RC = typename Lds<R>::C;
     constexpr int KS = 16, TM = 16 * MB, SA = TM + Lds<R>::SAPAD, SB = 17;
     constexpr int NA = TM * KS / 256, NB = GEMM_TN * KS / 256;   // elements per thread and slab
-    const GTile tl = tiles[tile0 + xcd_item(blockIdx.x, gridDim.x)];
-    const GGroup g = groups[tl.g];
     // columns are (box, rhs) pairs, rhs fastest: column n is box n / nr of the group in
-    // right-hand side n % nr (that rhs slab of the buffer)
+    // right-hand side n % nr (that rhs slab of the buffer).  A planned tile (GEMM_TN boxes)
+    // becomes nr launched tiles of GEMM_TN such columns; those past the group's last
+    // column exit at once.
+    const int bx = xcd_item(blockIdx.x, gridDim.x);
+    GTile tl = tiles[tile0 + bx / nr];
+    const GGroup g = groups[tl.g];
+    const int ncx = g.ncols * nr;
+    tl.n0 = tl.n0 * nr + GEMM_TN * (bx % nr);
+    if (tl.n0 >= ncx) return;
     const V* B = bufp<V>(bt, g.bbuf);
     V* C = bufp<V>(bt, g.cbuf);
     const int64_t sB = bt.stride[g.bbuf], sC = bt.stride[g.cbuf];
-    const int ncx = g.ncols * nr;
     // f32: two LDS slab buffers (one barrier per slab); f64 slabs are twice the size and
     // take twice the MFMA time per k, so one buffer and two barriers per slab
     constexpr int NBUF = sizeof(R) == 4 ? 2 : 1;
@@ -505,7 +510,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(
 
 template <class TA, class V, class R>
 void launch_gemm(int mb, int n, int nrhs, hipStream_t st, const DevPlan& d, const TA* opm, int t0) {
-    const dim3 grid(n, 1);
+    const dim3 grid(n * nrhs, 1);
 #define DDFMM_GEMM_MB(X)                                                                                \
     case X:                                                                                             \
         gemm_kernel<TA, V, R, X><<<grid, 256, 0, st>>>(d.groups, d.tiles, d.colB, d.colC, d.colA, opm, \

@@ -1494,23 +1494,4 @@ void build_plan(const Params& prm, const double* pos, int64_t N, const Mlib& mli
     b.run(pos, N, want_dump);
 }
 
-void expand_gemm_tiles(const HostPlan& plan, int nr, std::vector<GTile>& tiles,
-                       std::vector<std::pair<int, int>>& ranges) {
-    tiles.clear();
-    ranges.assign(plan.stages.size(), {0, 0});
-    for (size_t i = 0; i < plan.stages.size(); ++i) {
-        const Stage& s = plan.stages[i];
-        if (s.kind != ST_GEMM) continue;
-        const int b = (int)tiles.size();
-        for (int t = s.begin; t < s.end; ++t) {
-            const GTile& x = plan.tiles[t];
-            if (x.n0 != 0) continue;                // one entry per row tile of a group
-            const int64_t nc = (int64_t)plan.groups[x.g].ncols * nr;
-            if (nc > INT32_MAX) throw Error("GEMM group too large for the right-hand-side count");
-            for (int64_t n0 = 0; n0 < nc; n0 += GEMM_TN) tiles.push_back(GTile{x.g, x.m0, (int)n0, x.mb});
-        }
-        ranges[i] = {b, (int)tiles.size()};
-    }
-}
-
 }  // namespace ddfmm

@@ -57,11 +57,4 @@ struct HostPlan {
 void build_plan(const Params& prm, const double* pos, int64_t N, const Mlib& mlib,
                 HostPlan& plan, bool want_dump);
 
-// GEMM tiles for nr right-hand sides in one launch: every GEMM stage's row tiles over the
-// columns (box, rhs) of its groups, rhs fastest (ncols * nr columns, GEMM_TN per tile);
-// ranges[i] is the tile range of stage i (empty for other kinds).  nr = 1 reproduces
-// plan.tiles and the stages' own ranges.
-void expand_gemm_tiles(const HostPlan& plan, int nr, std::vector<GTile>& tiles,
-                       std::vector<std::pair<int, int>>& ranges);
-
 }  // namespace ddfmm

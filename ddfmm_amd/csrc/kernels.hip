@@ -905,8 +905,11 @@ hipError_t launch_stage_t(const DevPlan& d, const Stage& s, int nrhs, hipStream_
         if (nrhs == 1) {
             kapply_kernel<V, 1><<<dim3(gx, 1), KA_THREADS, 0, st>>>(d.ktasks, d.ksrcs, d.px, d.py, d.pz, d.bt,
                                                                     s.begin, n, nrhs, d.ktype);
-        } else {
-            kapply_kernel<V, 4><<<dim3(gx, (nrhs + 3) / 4), KA_THREADS, 0, st>>>(d.ktasks, d.ksrcs, d.px, d.py, d.pz,
+        } else if (nrhs <= 4) {
+            kapply_kernel<V, 4><<<dim3(gx, 1), KA_THREADS, 0, st>>>(d.ktasks, d.ksrcs, d.px, d.py, d.pz, d.bt,
+                                                                    s.begin, n, nrhs, d.ktype);
+        } else {    // G(t, s) evaluated once for 8 right-hand sides
+            kapply_kernel<V, 8><<<dim3(gx, (nrhs + 7) / 8), KA_THREADS, 0, st>>>(d.ktasks, d.ksrcs, d.px, d.py, d.pz,
                                                                                 d.bt, s.begin, n, nrhs, d.ktype);
         }
         break;

@@ -18,10 +18,27 @@ Engines are pluggable (``make_engine``) so the scheduler can be exercised withou
 from __future__ import annotations
 
 import collections
+import concurrent.futures
 import dataclasses
 import time
 
 import numpy as np
+
+_COPY_POOL = concurrent.futures.ThreadPoolExecutor(max_workers=8)
+_COPY_CHUNK = 1 << 23          # float64 elements (64 MiB) per copy task
+
+
+def _par_copy(dst: np.ndarray, src: np.ndarray):
+    """dst[:] = src for 1-D float64 arrays, in parallel chunks for large copies (numpy
+    releases the GIL while copying); returns when the whole copy is done."""
+    n = src.size
+    if n <= _COPY_CHUNK:
+        dst[:] = src
+        return
+    futs = [_COPY_POOL.submit(np.copyto, dst[i:i + _COPY_CHUNK], src[i:i + _COPY_CHUNK])
+            for i in range(0, n, _COPY_CHUNK)]
+    for f in futs:
+        f.result()
 
 
 @dataclasses.dataclass
@@ -82,10 +99,19 @@ class GpuEngine:
         return None
 
     def launch(self, slot, den: np.ndarray, token):
-        nb = den.shape[0]
+        self.launch_parts(slot, [den], token)
+
+    def launch_parts(self, slot, parts, token):
+        """Launch one evaluation whose RHS are the rows of the arrays in ``parts``, written
+        straight into the pinned staging buffer (no concatenated copy)."""
+        nb = sum(p.shape[0] for p in parts)
         nbytes = nb * self.N * 16
         host = np.ctypeslib.as_array((np.ctypeslib.ctypes.c_double * (2 * nb * self.N)).from_address(slot["hden"]))
-        host[:] = den.view(np.float64).reshape(-1)
+        c = 0
+        for p in parts:
+            k = 2 * p.shape[0] * self.N
+            _par_copy(host[c:c + k], np.ascontiguousarray(p).view(np.float64).reshape(-1))
+            c += k
         self.hip.copy_async(slot["dden"], slot["hden"], nbytes, 1, self.stream)
         self.w.eval_device(slot["dden"], slot["dval"], nb, self.stream)
         self.hip.copy_async(slot["hval"], slot["dval"], nbytes, 2, self.stream)
@@ -99,7 +125,9 @@ class GpuEngine:
             if s["busy"] is not None and self.hip.done(s["ev"]):
                 token, nb = s["busy"]
                 arr = np.ctypeslib.as_array((np.ctypeslib.ctypes.c_double * (2 * nb * self.N)).from_address(s["hval"]))
-                vals = arr.copy().view(np.complex128).reshape(nb, self.N)
+                flat = np.empty(arr.size, dtype=np.float64)
+                _par_copy(flat, arr)
+                vals = flat.view(np.complex128).reshape(nb, self.N)
                 s["busy"] = None
                 out.append((token, vals))
         return out
@@ -183,14 +211,16 @@ class Server:
                 r.error = f"request needs {r.nrhs} RHS, slot capacity is {eng.nrhs_slot}"
                 r.t_done = now
                 continue
-            den = np.concatenate([r.den for r in batch], axis=0)
             token = self.batches
             self.batches += 1
             for r in batch:
                 r.t_launch = now
                 r.batch = token
             self.inflight[token] = batch
-            eng.launch(slot, den, token)
+            if hasattr(eng, "launch_parts"):      # writes the requests into its staging directly
+                eng.launch_parts(slot, [r.den for r in batch], token)
+            else:
+                eng.launch(slot, np.concatenate([r.den for r in batch], axis=0), token)
 
     def _collect(self):
         for eng in self.engines.values():

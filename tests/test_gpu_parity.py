@@ -105,6 +105,21 @@ def test_multi_rhs_matches_single(oplib):
     w.close()
 
 
+def test_multi_rhs_eleven_mixed(oplib):
+    """11 right-hand sides: two eight-RHS kernel-apply blocks (the second partial), GEMM
+    tiles over (box, rhs) columns with 11 rhs per box, the V list per rhs slab."""
+    K, N = 4.0, 5000
+    pos = geometry.sphere(N, K, 8)
+    den = geometry.densities(N, 11, nrhs=11)
+    w = _wave(pos, K, oplib, nrhs_max=11, precision=MIXED)
+    many = w.eval(den)
+    for r in (0, 7, 8, 10):
+        assert np.array_equal(many[r], w.eval(den[r]))
+    three = w.eval(den[5:8])
+    assert np.array_equal(three, many[5:8])
+    w.close()
+
+
 def test_repeated_eval_is_fresh(oplib):
     K, N = 4.0, 4000
     pos = geometry.sphere(N, K, 3)

@@ -126,3 +126,40 @@ def test_stage_precision_classes_and_work(oplib, precision):
         assert {"lf_uc2ue_1", "lf_uc2ue_2", "hf_up_e3", "hf_dn_e1t"} <= names
     total = sum(s[1] for s in gemm)
     assert abs(total - st["gemm_flops"]) <= 1e-9 * st["gemm_flops"]
+
+
+def test_blocked_gemm_padding_stays_small(oplib):
+    """Column packing of the K-blocked GEMMs (child masks in U2U, direction bundles in the
+    width-1 L2L): on the 250k-point sphere the largest U2U and L2L launches execute at most
+    30% more MFMA work than the useful flops (a plain Gray-code order gave 1.3-1.6 at
+    width 2), and the width-1 L2L bundles leave the LF reduction a few slots per child."""
+    pos = geometry.sphere(250_000, 16.0, 2)
+    prm = _lib.Params(K=16.0, ACCU=2, NPQ=4, ptsmax=100, maxlevel=12, numC=1, kernel=0, nrhs_max=1,
+                      precision=1)
+    h = ctypes.c_void_p()
+    _lib.check(_lib.lib().ddfmm_create(-1, ctypes.byref(h)))
+    try:
+        _lib.check(_lib.lib().ddfmm_setup(h, pos.ctypes.data, len(pos), ctypes.byref(prm), oplib[0].encode(),
+                                          oplib[1].encode()))
+        cnt = ctypes.c_int()
+        _lib.check(_lib.lib().ddfmm_stage_times(h, 0, None, None, None, None, ctypes.byref(cnt)))
+        n = cnt.value
+        names = ctypes.create_string_buffer(24 * n)
+        work = (ctypes.c_double * n)()
+        kinds = (ctypes.c_int * n)()
+        ex = (ctypes.c_double * n)()
+        _lib.check(_lib.lib().ddfmm_stage_times(h, n, names, None, work, kinds, ctypes.byref(cnt)))
+        _lib.check(_lib.lib().ddfmm_stage_exec(h, n, ex, ctypes.byref(cnt)))
+        st = [(names.raw[24 * i:24 * i + 24].split(b"\0")[0].decode(), work[i], ex[i], kinds[i] % 16)
+              for i in range(n)]
+    finally:
+        _lib.lib().ddfmm_destroy(h)
+    for name in ("hf_u2u", "hf_l2l"):
+        big = max((s for s in st if s[0] == name), key=lambda s: s[1])
+        assert big[1] > 0 and big[2] / big[1] <= 1.3, (name, big[2] / big[1])
+    # width-1 L2L: useful flops 8 * 152 * kb per (LF child, present direction), one slot of
+    # 152 elements per (LF child, bundle); kb (the width-1 solve size) is at most 128, so
+    # flops / (8 * 128) / slot elements bounds the mean number of directions per slot
+    l2l_w1 = max(s[1] for s in st if s[0] == "hf_l2l")
+    lf_red = sum(s[1] for s in st if s[0] == "lf_reduce")
+    assert lf_red > 0 and l2l_w1 / (8.0 * 128) / lf_red >= 3.0

@@ -291,10 +291,35 @@ struct Builder {
                   nm.find("hf_up_e") != std::string::npos || nm.find("hf_dn_e") != std::string::npos)
                      ? PREC_SOLVE : PREC_TRANSLATION;
         if (kind == ST_GEMM) {
+            resplit_rows(b, e);
+            s.end = (int)pl.tiles.size();
             push_gemm_stages(pl.stages, s);
             return;
         }
         pl.stages.push_back(s);
+    }
+    // A GEMM stage with fewer than MIN_GEMM_TILES tiles (an LF level's M2M over a few
+    // hundred boxes) would leave most CUs idle: its groups' rows are split into
+    // proportionally more, shorter tiles (the tiles of the range are the last ones added).
+    static constexpr int MIN_GEMM_TILES = 512;
+    void resplit_rows(int b, int e) {
+        if (e - b >= MIN_GEMM_TILES || e != (int)pl.tiles.size() || e <= b) return;
+        std::vector<int> gs;
+        for (int t = b; t < e; ++t)
+            if (gs.empty() || gs.back() != pl.tiles[t].g) gs.push_back(pl.tiles[t].g);
+        const int f = (MIN_GEMM_TILES + (e - b) - 1) / (e - b);
+        pl.tiles.resize(b);
+        for (int gi : gs) {
+            const GGroup& g = pl.groups[gi];
+            const int mbt = (g.M + 15) / 16;
+            const int nmt = std::min(mbt, (mbt + GEMM_MB - 1) / GEMM_MB * f);
+            const int nt = (g.ncols + GEMM_TN - 1) / GEMM_TN;
+            for (int a = 0, b0 = 0; a < nmt; ++a) {
+                const int mb = (mbt - b0 + (nmt - a) - 1) / (nmt - a);
+                for (int c = 0; c < nt; ++c) pl.tiles.push_back(GTile{gi, 16 * b0, c * GEMM_TN, mb});
+                b0 += mb;
+            }
+        }
     }
     int64_t gen(int64_t toff, int m, const double ts[3], int tneg, int64_t soff, int n,
                 const double ss[3]) {

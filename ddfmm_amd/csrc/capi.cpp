@@ -140,6 +140,7 @@ int ddfmm_setup(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_params* 
         q.ACCU = p->ACCU; q.NPQ = p->NPQ; q.ptsmax = p->ptsmax; q.maxlevel = p->maxlevel;
         q.numC = p->numC; q.ktype = p->kernel;
         q.fuse_solves = p->precision != DDFMM_FP64;
+        q.nrhs_max = p->nrhs_max;
         c->nrhs_max = p->nrhs_max;
         c->precision = p->precision;
         Mlib mlib;

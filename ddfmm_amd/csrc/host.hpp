@@ -54,6 +54,7 @@ struct Params {
     double ctr[3] = {0, 0, 0};
     int ACCU = 2, NPQ = 4, ptsmax = 100, maxlevel = 12, numC = 1, ktype = 0;
     bool fuse_solves = false;   // apply check->equivalent solves as single precomputed matrices
+    int nrhs_max = 1;           // right-hand sides an evaluation may carry (GEMM launch sizing)
 };
 
 class Error : public std::runtime_error {

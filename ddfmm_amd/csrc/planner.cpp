@@ -298,16 +298,19 @@ struct Builder {
         }
         pl.stages.push_back(s);
     }
-    // A GEMM stage with fewer than MIN_GEMM_TILES tiles (an LF level's M2M over a few
-    // hundred boxes) would leave most CUs idle: its groups' rows are split into
-    // proportionally more, shorter tiles (the tiles of the range are the last ones added).
-    static constexpr int MIN_GEMM_TILES = 512;
+    // A GEMM stage whose launch (tiles x right-hand sides) has fewer than MIN_GEMM_TILES
+    // tiles (an LF level's M2M over a few hundred boxes) would leave most CUs idle: its
+    // groups' rows are split into proportionally more, shorter tiles (the tiles of the
+    // range are the last ones added).
+    static constexpr int MIN_GEMM_TILES = 2048;
     void resplit_rows(int b, int e) {
-        if (e - b >= MIN_GEMM_TILES || e != (int)pl.tiles.size() || e <= b) return;
+        if ((int64_t)(e - b) * std::max(1, prm.nrhs_max) >= MIN_GEMM_TILES || e != (int)pl.tiles.size() || e <= b)
+            return;
         std::vector<int> gs;
         for (int t = b; t < e; ++t)
             if (gs.empty() || gs.back() != pl.tiles[t].g) gs.push_back(pl.tiles[t].g);
-        const int f = (MIN_GEMM_TILES + (e - b) - 1) / (e - b);
+        const int64_t want = MIN_GEMM_TILES / std::max(1, prm.nrhs_max);
+        const int f = (int)((want + (e - b) - 1) / (e - b));
         pl.tiles.resize(b);
         for (int gi : gs) {
             const GGroup& g = pl.groups[gi];

@@ -160,6 +160,7 @@ def test_blocked_gemm_padding_stays_small(oplib):
     # width-1 L2L: useful flops 8 * 152 * kb per (LF child, present direction), one slot of
     # 152 elements per (LF child, bundle); kb (the width-1 solve size) is at most 128, so
     # flops / (8 * 128) / slot elements bounds the mean number of directions per slot
-    l2l_w1 = max(s[1] for s in st if s[0] == "hf_l2l")
+    last_red = max(i for i, s in enumerate(st) if s[0] == "hf_reduce")      # width 1 comes last
+    l2l_w1 = sum(s[1] for s in st[last_red:] if s[0] == "hf_l2l")
     lf_red = sum(s[1] for s in st if s[0] == "lf_reduce")
     assert lf_red > 0 and l2l_w1 / (8.0 * 128) / lf_red >= 3.0

@@ -69,24 +69,54 @@ def test_direction_algebra(W):
 
 
 # ---------------------------------------------------------------- operators (A6/A7/A12/A19)
-def test_lf_pinv_and_far_field(ops):
-    ld, _ = ops
-    for W in (0.5, 0.25):
+# Far-field reproduction bounds of the LF equivalent densities per kernel and width.
+# e^{ikr}/r solves the Helmholtz equation away from the sources, so check-surface values
+# determine the far field and the ACCU = 2 surfaces reproduce it to ~1e-6.  e^{ikr} solves
+# no PDE: the equivalent densities still fit the check surface to ~1e-9 (the pinv is
+# right), but the far field of sources inside the box is not a function of those values,
+# so the reproduction error is a property of the kernel-independent method at P = 6
+# (1.9e-2 at W = 1/2, 2.8e-4 at W = 1/4, 4.2e-5 at W = 1/8 with these samples).
+LF_FAR_BOUND = {0: {0.5: 1e-5, 0.25: 1e-5, 0.125: 1e-5},
+                1: {0.5: 3e-2, 0.25: 1e-3, 0.125: 1e-4}}
+
+
+@pytest.mark.parametrize("ktype", [0, 1])
+def test_lf_pinv_and_far_field(ops, ops_expr, ktype):
+    ld, _ = ops if ktype == 0 else ops_expr
+    far_err = {}
+    for W in (0.5, 0.25, 0.125):
         e = ld[W]
         uep, ucp = e["uep"].T, e["ucp"].T
         V, invs, UH = e["uc2ue"]
-        G = R.kernel(ucp, uep, 0)
+        G = R.kernel(ucp, uep, ktype)
         pinv = V @ (invs * UH)
         assert rel(G @ pinv @ G, G) < 1e-7
         # equivalent density reproduces the far field of random sources inside the box
         rng = np.random.default_rng(1)
         src = rng.uniform(-W / 2, W / 2, (40, 3))
         f = rng.standard_normal(40) + 1j * rng.standard_normal(40)
-        eq = pinv @ (R.kernel(ucp, src, 0) @ f)
+        chk = R.kernel(ucp, src, ktype) @ f
+        eq = pinv @ chk
+        assert rel(G @ eq, chk) < 1e-7                      # exact check-surface fit
         d = rng.standard_normal((30, 3))
         far = d / np.linalg.norm(d, axis=1, keepdims=True) * (2.0 * W) + rng.uniform(-0.2, 0.2, (30, 3)) * W
         far = far * 1.6  # well separated targets (distance >= 2W)
-        assert rel(R.kernel(far, uep, 0) @ eq, R.kernel(far, src, 0) @ f) < 1e-5
+        far_err[W] = rel(R.kernel(far, uep, ktype) @ eq, R.kernel(far, src, ktype) @ f)
+        assert far_err[W] < LF_FAR_BOUND[ktype][W], (W, far_err[W])
+    if ktype == 1:      # the method limit shrinks with the box width (kW = pi, pi/2, pi/4)
+        assert far_err[0.5] > far_err[0.25] > far_err[0.125]
+
+
+def test_expr_operators_differ_from_helm(ops, ops_expr):
+    """The e^{ikr} library is generated with its own kernel: its pinv does not invert the
+    e^{ikr}/r check matrix (a library written with the wrong kernel fails here)."""
+    for W in (0.5, 0.25):
+        e0, e1 = ops[0][W], ops_expr[0][W]
+        uep, ucp = e1["uep"].T, e1["ucp"].T
+        V, invs, UH = e1["uc2ue"]
+        G0 = R.kernel(ucp, uep, 0)
+        assert rel(G0 @ (V @ (invs * UH)) @ G0, G0) > 1e-3
+        assert not np.allclose(e0["ue2dc"][(2, 0, 0)], e1["ue2dc"][(2, 0, 0)])
 
 
 def test_lf_m2l_fft_equals_dense(ops):
@@ -107,24 +137,26 @@ def test_lf_m2l_fft_equals_dense(ops):
         assert rel(y, dense) < 1e-12
 
 
+@pytest.mark.parametrize("ktype", [0, 1])
 @pytest.mark.parametrize("W", [1, 2])
-def test_hf_directional_reproduction(ops, W):
-    """Directional equivalent densities reproduce the far field inside the wedge."""
-    _, hd = ops
+def test_hf_directional_reproduction(ops, ops_expr, W, ktype):
+    """Directional equivalent densities reproduce the far field inside the wedge (both
+    kernels: the directional low-rank property holds for e^{ikr} as for e^{ikr}/r)."""
+    _, hd = ops if ktype == 0 else ops_expr
     rng = np.random.default_rng(3)
     for n, e in hd[float(W)].items():
         uep, ucp = e["uep"].T, e["ucp"].T
         E1, E2, E3 = e["uc2ue"]
         src = rng.uniform(-W / 2, W / 2, (60, 3))
         f = rng.standard_normal(60) + 1j * rng.standard_normal(60)
-        eq = E1 @ (E2 @ (E3 @ (R.kernel(ucp, src, 0) @ f)))
+        eq = E1 @ (E2 @ (E3 @ (R.kernel(ucp, src, ktype) @ f)))
         C = 4 * W
         D = W * W + W
         ang = (np.array(n[:2], float) + rng.uniform(-1, 1, (40, 2))) / C * math.pi / 4
         v = np.stack([np.tan(ang[:, 0]), np.tan(ang[:, 1]), np.ones(40)], 1)
         v /= np.linalg.norm(v, axis=1, keepdims=True)
         trg = v * rng.uniform(D + W, 4 * D, (40, 1))
-        assert rel(R.kernel(trg, uep, 0) @ eq, R.kernel(trg, src, 0) @ f) < 2e-5
+        assert rel(R.kernel(trg, uep, ktype) @ eq, R.kernel(trg, src, ktype) @ f) < 2e-5
 
 
 def test_serialize_roundtrip(tmp_path):

@@ -2,13 +2,17 @@
 """Throughput of the directional-FMM evaluation pass on one MI355X.
 
 One step = one Wave3d::eval over the workload (densities and potentials resident in
-HBM, eval_device).  Default workload: sphere of diameter 32 wavelengths, N = 1,000,000
-area-uniform points (seed 1), one complex-normal density vector, kernel exp(ikr)/r,
-ACCU=2 (P=6), NPQ=4, ptsmax=100.  Metric: M point-evaluations/s per RHS, reported
+HBM, eval_device).  Default workload: the largest single-GPU configuration, the
+perturbed ellipsoid with semi-axes 32 x 8 x 8 wavelengths (radial factor
+1 + 0.1 sin(3 theta) cos(5 phi)), N = 4,000,000 area-uniform points (seed 3), 8
+complex-normal density vectors, kernel exp(ikr)/r, K = 128, 8^3 cells, ptsmax = 400,
+ACCU=2 (P=6), NPQ=4, mixed precision.  Metric: M point-evaluations/s per RHS, reported
 with the relative L2 error at 1000 random targets against an FP64 direct sum.
 
-Prints one JSON line.  Also reports the roofline of the dominant kernel (HIP events on
-the evaluation stream) and a CPU baseline (the oracle's numpy FMM on a bounded sample).
+The same JSON line carries the plain-FP64 time of the same workload (the mixed/FP64
+ratio), the 1M-point sphere (1 RHS) and the 250k-point sphere (64 RHS) in both
+precisions, the roofline of the dominant kernel (HIP events on the stages' streams), a
+serving leg (p50/p99 latency) and a CPU baseline on a bounded sample.
 """
 from __future__ import annotations
 
@@ -64,7 +68,7 @@ def serve_bench(names, qps, count, slot, precision, seed=4):
     """Poisson request stream over resident geometries (continuous batching)."""
     from ddfmm_amd import geometry, mlib_gen
     from ddfmm_amd.serve import GpuEngine, Server, latency_stats, poisson_requests
-    from ddfmm_amd.wave3d import FP64, MIXED
+    from ddfmm_amd.wave3d import FP64, FP64_FUSED, MIXED
 
     ops = mlib_gen.ensure_library(os.path.join(ROOT, "ddfmm_amd", "_ops"), ktype=0, accu=2, NPQ=4,
                                   hf_max_W=4, lf_min_W=2.0 ** -3)
@@ -74,7 +78,8 @@ def serve_bench(names, qps, count, slot, precision, seed=4):
         wl = geometry.WORKLOADS[key]
         return GpuEngine(key, pts[key], {"-wave3d_ACCU": 2, "-wave3d_K": wl["K"], "-wave3d_ptsmax": wl["ptsmax"],
                                          "-wave3d_maxlevel": 14}, ops, slot,
-                         MIXED if precision == "mixed" else FP64, numC=wl.get("numC", 1))
+                         {"mixed": MIXED, "fp64": FP64, "fp64_fused": FP64_FUSED}[precision],
+                         numC=wl.get("numC", 1))
 
     srv = Server(make, lambda key: 0, hbm_budget=200 << 30)
     t0 = time.perf_counter()
@@ -98,58 +103,66 @@ def serve_bench(names, qps, count, slot, precision, seed=4):
     return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="sphere32_1m")
-    ap.add_argument("--kernel", type=int, default=0)
-    ap.add_argument("--nrhs", type=int, default=None)
-    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed", "fp64_fused"])
-    ap.add_argument("--calib-copy", action="store_true",
-                    help="one 256 MiB device-to-device copy (blit kernel) to calibrate PMC byte counters")
-    ap.add_argument("--chk", type=int, default=1000)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--serve", default="sphere8_60k,sphere16_250k_x64,sphere32_1m,ellipsoid_4m_x8",
-                    help="comma-separated geometries for the serving leg ('' to skip)")
-    ap.add_argument("--serve-qps", type=float, default=10.0)
-    ap.add_argument("--serve-requests", type=int, default=48)
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(one device share per process, no collectives).", file=sys.stderr)
-        sys.exit(2)
+KNAME = {
+    "gemm_translation": lambda vt, rc: "gemm_kernel<float2,float2,float>" if vt == "float2"
+    else "gemm_kernel<double2,double2,double>",
+    "gemm_solve": lambda vt, rc: "gemm_kernel<double2,float2,double>" if vt == "float2"
+    else "gemm_kernel<double2,double2,double>",
+    "kapply": lambda vt, rc: f"kapply_kernel<{vt},{rc}>",
+    "vlist": lambda vt, rc: f"vlist_acc_kernel<{vt},{rc}>",
+    "fft_fwd": lambda vt, rc: f"fft_fwd_kernel<{vt}>",
+    "reduce": lambda vt, rc: f"reduce_kernel<{vt}>",
+}
 
+
+def rhs_block(nrhs: int) -> int:
+    """Right-hand sides one kernel-apply / V-list block carries (as the launcher picks it)."""
+    return 1 if nrhs == 1 else 4 if nrhs <= 4 else 8
+
+
+def measure(workload, precision, kernel, nrhs, steps, warmup, nchk, detail=False, calib=False, tune=True,
+            graph=True):
+    """Set up one Wave3d on the workload, time `steps` evaluations (after `warmup`) with
+    densities and potentials resident in HBM, and check 1000 targets against the FP64
+    direct sum.  With detail, also per-stage HIP-event times and the dominant kernel's
+    roofline."""
     from ddfmm_amd import geometry, mlib_gen
     from ddfmm_amd._lib import Hip
     from ddfmm_amd.wave3d import FP64, FP64_FUSED, MIXED, Kernel3d, Mlib3d, Wave3d, direct_sum
 
-    wl = dict(geometry.WORKLOADS[args.workload])
-    nrhs = args.nrhs or wl["nrhs"]
-    ops = mlib_gen.ensure_library(os.path.join(ROOT, "ddfmm_amd", "_ops"), ktype=args.kernel, accu=2, NPQ=4,
+    wl = dict(geometry.WORKLOADS[workload])
+    nrhs = nrhs or wl["nrhs"]
+    ops = mlib_gen.ensure_library(os.path.join(ROOT, "ddfmm_amd", "_ops"), ktype=kernel, accu=2, NPQ=4,
                                   hf_max_W=4, lf_min_W=2.0 ** -3)
     t0 = time.perf_counter()
-    pos = geometry.workload_points(args.workload)
+    pos = geometry.workload_points(workload)
     den = geometry.densities(len(pos), wl["seed"], nrhs=nrhs)
     N = len(pos)
-    log(f"workload {args.workload}: N={N} nrhs={nrhs} kernel={args.kernel} ({time.perf_counter() - t0:.1f}s)")
-
+    log(f"{workload} {precision}: N={N} nrhs={nrhs} kernel={kernel} ({time.perf_counter() - t0:.1f}s)")
     ml = Mlib3d()
     ml.setup({"-mlib3d_ldname": ops[0], "-mlib3d_hdname": ops[1]})
     w = Wave3d()
-    w.posptr, w.mlibptr, w.kernel = pos, ml, Kernel3d(args.kernel)
+    w.posptr, w.mlibptr, w.kernel = pos, ml, Kernel3d(kernel)
     w.geomprtn = np.zeros((wl.get("numC", 1),) * 3, dtype=np.int32)
     t0 = time.perf_counter()
     w.setup({"-wave3d_ACCU": 2, "-wave3d_NPQ": 4, "-wave3d_K": wl["K"], "-wave3d_ptsmax": wl["ptsmax"],
              "-wave3d_maxlevel": 14}, nrhs_max=nrhs,
-            precision={"mixed": MIXED, "fp64": FP64, "fp64_fused": FP64_FUSED}[args.precision])
+            precision={"mixed": MIXED, "fp64": FP64, "fp64_fused": FP64_FUSED}[precision])
+    setup_s = time.perf_counter() - t0
     st = w.stats()
-    log(f"setup {time.perf_counter() - t0:.1f}s: leaves={st['nleaf']} m2l_pairs={st['m2l_pairs']} "
+    log(f"setup {setup_s:.1f}s: leaves={st['nleaf']} m2l_pairs={st['m2l_pairs']} "
         f"v_pairs={st['v_pairs']} device={st['device_bytes'] / 2**30:.2f} GiB")
-
+    tuned = None
+    if tune:
+        t0 = time.perf_counter()
+        tuned = w.autotune(nrhs, reps=1)
+        tuned["seconds"] = time.perf_counter() - t0
+        log(f"autotune {tuned['seconds']:.1f}s: GEMM stages {tuned['gemm_ms_planner']:.3f} ms (planner tiles) -> "
+            f"{tuned['gemm_ms_tuned']:.3f} ms (tuned), heights {tuned['tiles']}")
+    if graph:
+        w.set_graph(True)
     hip = Hip()
-    if args.calib_copy:
+    if calib:
         nb = 256 << 20
         a, b = hip.malloc(nb), hip.malloc(nb)
         hip.copy_d2d(b, a, nb)
@@ -159,27 +172,38 @@ def main():
     den2 = np.ascontiguousarray(np.atleast_2d(den))
     dd = hip.to_device(den2)
     vd = hip.malloc(den2.nbytes)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         w.eval_device(dd, vd, nrhs)
     hip.sync()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         w.eval_device(dd, vd, nrhs)
     hip.sync()
-    dt = (time.perf_counter() - t0) / args.steps
-    log(f"{args.steps} steps: {dt * 1e3:.3f} ms/step")
-
-    # accuracy at random targets vs the FP64 direct sum
+    dt = (time.perf_counter() - t0) / steps
+    log(f"{steps} steps: {dt * 1e3:.3f} ms/step")
     val = hip.to_host(vd, np.empty_like(den2))
-    chk = geometry.check_targets(N, args.chk, wl["seed"])
-    exact = direct_sum(pos[chk], pos, den2, args.kernel)
+    chk = geometry.check_targets(N, nchk, wl["seed"])
+    exact = direct_sum(pos[chk], pos, den2, kernel)
     err = float(np.linalg.norm(val[:, chk] - exact) / np.linalg.norm(exact))
     log(f"rel.err vs FP64 direct at {len(chk)} targets: {err:.3e}")
+    out = {"workload": workload, "precision": precision, "n_points": N, "nrhs": nrhs, "K": wl["K"],
+           "ptsmax": wl["ptsmax"], "kernel": "exp(ikr)/r" if kernel == 0 else "exp(ikr)",
+           "ms_per_step": dt * 1e3, "value": N * nrhs / dt / 1e6, "rel_err_vs_direct": err,
+           "n_check": int(len(chk)), "setup_s": setup_s, "device_GiB": st["device_bytes"] / 2**30,
+           "autotune": tuned, "graph": bool(graph)}
+    if detail:
+        out.update(stage_detail(w, st, dd, vd, nrhs, precision, workload))
+    hip.free(dd)
+    hip.free(vd)
+    w.close()
+    return out
 
-    # per-kernel time from HIP events on the evaluation stream
+
+def stage_detail(w, st, dd, vd, nrhs, precision, workload):
+    """Per-stage HIP-event times on the stages' own streams, the kernel classes and the
+    roofline of the dominant class (work per launch / average launch duration)."""
     w.set_timing(True)
-    per = {}
-    per_stage = {}
+    per, per_stage = {}, {}
     reps = 3
     for _ in range(reps):
         w.eval_device(dd, vd, nrhs)
@@ -199,28 +223,26 @@ def main():
     w.set_timing(False)
     log("top stages (ms, work/ms): " + ", ".join(
         f"{n}={v[0]:.3f}({v[1] / max(v[0], 1e-9) / 1e9:.2f}G/ms)" for n, v in
-        sorted(per_stage.items(), key=lambda kv: -kv[1][0])[:12]))
+        sorted(per_stage.items(), key=lambda kv: -kv[1][0])[:14]))
+    log("per kernel class (ms/eval, launches): " + ", ".join(
+        f"{k}={v['ms']:.3f}/{v['launches']}" for k, v in sorted(per.items(), key=lambda kv: -kv[1]["ms"])))
     dom = max(per, key=lambda k: per[k]["ms"])
     t_dom = per[dom]["ms"] * 1e-3
     launches = per[dom]["launches"]
-    log("per kernel class (ms/eval, launches): " + ", ".join(
-        f"{k}={v['ms']:.3f}/{v['launches']}" for k, v in sorted(per.items(), key=lambda kv: -kv[1]["ms"])))
-    mixed = args.precision == "mixed"
+    mixed = precision == "mixed"
     esz = 8 if mixed else 16
     vt = "float2" if mixed else "double2"
-    kname = {"gemm_translation": "gemm_kernel<float2,float2,float>" if mixed else "gemm_kernel<double2,double2,double>",
-             "gemm_solve": "gemm_kernel<double2,float2,double>" if mixed else "gemm_kernel<double2,double2,double>",
-             "kapply": f"kapply_kernel<{vt},{1 if nrhs == 1 else 4}>", "vlist": f"vlist_acc_kernel<{vt}>",
-             "fft_fwd": f"fft_fwd_kernel<{vt}>", "reduce": f"reduce_kernel<{vt}>"}.get(dom, dom)
+    kname = KNAME.get(dom, lambda vt, rc: dom)(vt, rhs_block(nrhs))
     if dom in ("gemm_translation", "gemm_solve"):
         peak = PEAK["fp32"] if (mixed and dom == "gemm_translation") else PEAK["fp64"]
-        roof = {"bound": "mfma", "achieved": per[dom]["work"] / t_dom / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s"}
+        roof = {"bound": "mfma", "achieved": per[dom]["work"] / t_dom / 1e12, "peak": peak / 1e12,
+                "unit": "TFLOP/s"}
         work_desc = "useful complex-GEMM flops 8*M*K*N over the class's groups"
     elif dom == "kapply":
         work = per[dom]["work"] / nrhs * (20 + 8 * nrhs)
         peak = PEAK["fp32"] if mixed else PEAK["fp64"]
-        roof = {"bound": "mfma", "achieved": work / t_dom / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s"}
-        work_desc = "target-source interactions x (20 + 8*nrhs) flops (VALU)"
+        roof = {"bound": "valu", "achieved": work / t_dom / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s"}
+        work_desc = "target-source interactions x (20 + 8*nrhs) flops on the vector ALU"
     elif dom == "reduce":
         work = per[dom]["work"] * esz
         roof = {"bound": "hbm", "achieved": work / t_dom / 1e9, "peak": PEAK["hbm"] / 1e9, "unit": "GB/s"}
@@ -234,7 +256,7 @@ def main():
         work_desc = "n/a"
     roof["frac"] = roof["achieved"] / roof["peak"]
     roof["traffic"] = None
-    tfile = os.path.join(ROOT, "profiles", f"{args.workload}_{args.precision}_traffic.json")
+    tfile = os.path.join(ROOT, "profiles", f"{workload}_{precision}_traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
             roof["traffic"] = json.load(f).get(kname)
@@ -243,9 +265,56 @@ def main():
     roof["launches_per_eval"] = launches
     roof["avg_launch_us"] = t_dom / max(launches, 1) * 1e6
     roof["work"] = work_desc
-    hip.free(dd)
-    hip.free(vd)
-    w.close()
+    return {"roofline": roof, "per_kernel_ms": {k: round(v["ms"], 4) for k, v in per.items()},
+            "per_stage_ms": {k: round(v[0], 4) for k, v in per_stage.items()}}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="ellipsoid_4m_x8",
+                    help="headline workload (default: the largest single-GPU configuration)")
+    ap.add_argument("--kernel", type=int, default=0)
+    ap.add_argument("--nrhs", type=int, default=None)
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed", "fp64_fused"])
+    ap.add_argument("--no-fp64-leg", action="store_true",
+                    help="skip the plain-FP64 run of the headline workload (the mixed/FP64 ratio)")
+    ap.add_argument("--extra", default="sphere32_1m,sphere16_250k_x64",
+                    help="comma-separated further workloads timed in mixed and plain FP64 ('' to skip)")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="one 256 MiB device-to-device copy (blit kernel) to calibrate PMC byte counters")
+    ap.add_argument("--chk", type=int, default=1000)
+    ap.add_argument("--no-autotune", action="store_true", help="keep the planner's GEMM tiling")
+    ap.add_argument("--no-graph", action="store_true", help="issue the stages one by one (no graph replay)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve", default="sphere8_60k,sphere16_250k_x64,sphere32_1m,ellipsoid_4m_x8",
+                    help="comma-separated geometries for the serving leg ('' to skip)")
+    ap.add_argument("--serve-qps", type=float, default=10.0)
+    ap.add_argument("--serve-requests", type=int, default=48)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(one device share per process, no collectives).", file=sys.stderr)
+        sys.exit(2)
+
+    from ddfmm_amd import mlib_gen
+
+    tg = dict(tune=not args.no_autotune, graph=not args.no_graph)
+    head = measure(args.workload, args.precision, args.kernel, args.nrhs, args.steps, args.warmup, args.chk,
+                   detail=True, calib=args.calib_copy, **tg)
+    fp64 = None
+    if not args.no_fp64_leg and args.precision != "fp64":
+        fp64 = measure(args.workload, "fp64", args.kernel, args.nrhs, max(2, args.steps // 2), 1, args.chk, **tg)
+    extra = {}
+    for wname in [x for x in args.extra.split(",") if x and x != args.workload]:
+        m = measure(wname, "mixed", args.kernel, None, max(2, args.steps // 2), 1, args.chk, **tg)
+        f = measure(wname, "fp64", args.kernel, None, max(2, args.steps // 2), 1, args.chk, **tg)
+        extra[wname] = {"nrhs": m["nrhs"], "n_points": m["n_points"], "mixed_ms": m["ms_per_step"],
+                        "mixed_value": m["value"], "mixed_rel_err": m["rel_err_vs_direct"],
+                        "fp64_ms": f["ms_per_step"], "fp64_value": f["value"], "fp64_rel_err": f["rel_err_vs_direct"],
+                        "mixed_vs_fp64": f["ms_per_step"] / m["ms_per_step"]}
 
     serve = None
     if args.serve:
@@ -265,29 +334,38 @@ def main():
                            args.kernel)
         log(f"cpu baseline {cpu['value']:.3f} M/s on {cpu['cores']} cores")
 
-    value = N * nrhs / dt / 1e6
+    err = head["rel_err_vs_direct"]
+    cfg = {"workload": args.workload, "n_points": head["n_points"], "nrhs": head["nrhs"],
+           "kernel": head["kernel"], "K": head["K"], "ACCU": 2, "NPQ": 4, "ptsmax": head["ptsmax"],
+           "precision": args.precision, "rel_err_vs_direct": err, "n_check": head["n_check"],
+           "meets_tolerance": bool(err <= 1e-5), "device_GiB": round(head["device_GiB"], 2),
+           "graph_replay": head["graph"], "autotune": head["autotune"]}
+    if fp64 is not None:
+        cfg.update({"fp64_ms_per_step": fp64["ms_per_step"], "fp64_value": fp64["value"],
+                    "fp64_rel_err_vs_direct": fp64["rel_err_vs_direct"],
+                    "mixed_vs_fp64": fp64["ms_per_step"] / head["ms_per_step"]})
     out = {
         "metric": "M point-evals/s·RHS at rel.err<=1e-5 vs FP64 direct",
-        "value": value,
+        "value": head["value"],
         "unit": "M point-evals/s·RHS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": dt * 1e3,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"mixed": "fp32+fp64 (FP32: translations with the HF solves folded in, M2L, near field, V list; FP64: LF check-to-equivalent solves)", "fp64": "fp64",
+        "dtype": {"mixed": "fp32+fp64 (FP32: translations with the HF solves folded in, M2L, near field, V list; "
+                           "FP64: LF check-to-equivalent solves, coordinate differences)", "fp64": "fp64",
                   "fp64_fused": "fp64 (fused solves)"}[args.precision],
         "data": "synthetic",
-        "config": {"workload": args.workload, "n_points": N, "nrhs": nrhs, "kernel": "exp(ikr)/r" if args.kernel == 0
-                   else "exp(ikr)", "K": wl["K"], "ACCU": 2, "NPQ": 4, "ptsmax": wl["ptsmax"],
-                   "precision": args.precision, "rel_err_vs_direct": err, "n_check": int(len(chk)),
-                   "meets_tolerance": bool(err <= 1e-5)},
-        "roofline": roof,
+        "config": cfg,
+        "roofline": head["roofline"],
         "cpu_baseline": cpu,
         "serve": serve,
-        "per_kernel_ms": {k: round(v["ms"], 4) for k, v in per.items()},
+        "configs": extra,
+        "per_kernel_ms": head["per_kernel_ms"],
+        "per_stage_ms": head["per_stage_ms"],
     }
     print(json.dumps(out), flush=True)
 

@@ -12,7 +12,8 @@ LIB_PATH = os.path.join(HERE, "libddfmm.so")
 EXPORTS = ["ddfmm_last_error", "ddfmm_create", "ddfmm_destroy", "ddfmm_setup", "ddfmm_eval",
            "ddfmm_eval_device", "ddfmm_direct_sum", "ddfmm_kernel_matrix", "ddfmm_get_stats",
            "ddfmm_dump_lists", "ddfmm_read_buffer", "ddfmm_set_timing", "ddfmm_stage_times",
-           "ddfmm_stage_exec"]
+           "ddfmm_stage_exec", "ddfmm_set_graph", "ddfmm_mlib_load", "ddfmm_mlib_free", "ddfmm_mlib_widths",
+           "ddfmm_setup_mlib", "ddfmm_autotune", "ddfmm_set_tiles", "ddfmm_get_tiles"]
 
 
 class Params(ctypes.Structure):
@@ -56,6 +57,11 @@ def lib():
     L.ddfmm_destroy.argtypes = [vp]
     L.ddfmm_setup.argtypes = [vp, vp, i64, ctypes.POINTER(Params), ctypes.c_char_p, ctypes.c_char_p]
     L.ddfmm_eval.argtypes = [vp, vp, vp, ci]
+    L.ddfmm_mlib_load.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp)]
+    L.ddfmm_mlib_free.argtypes = [vp]
+    L.ddfmm_mlib_widths.argtypes = [vp, ci, ctypes.POINTER(cd), ctypes.POINTER(ci), ctypes.POINTER(cd),
+                                    ctypes.POINTER(ci)]
+    L.ddfmm_setup_mlib.argtypes = [vp, vp, i64, ctypes.POINTER(Params), vp]
     L.ddfmm_eval_device.argtypes = [vp, vp, vp, ci, vp]
     L.ddfmm_direct_sum.argtypes = [ci, vp, ci, vp, ci, vp, ci, vp]
     L.ddfmm_kernel_matrix.argtypes = [ci, vp, ci, vp, ci, vp]
@@ -63,6 +69,11 @@ def lib():
     L.ddfmm_dump_lists.argtypes = [vp, ctypes.c_char_p, i64, ctypes.POINTER(i64)]
     L.ddfmm_read_buffer.argtypes = [vp, ci, ci, i64, i64, vp]
     L.ddfmm_set_timing.argtypes = [vp, ci]
+    L.ddfmm_set_graph.argtypes = [vp, ci]
+    L.ddfmm_autotune.argtypes = [vp, ci, ci, ci, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(cd),
+                                 ctypes.POINTER(cd)]
+    L.ddfmm_set_tiles.argtypes = [vp, ci, ctypes.POINTER(ci)]
+    L.ddfmm_get_tiles.argtypes = [vp, ci, ctypes.POINTER(ci), ctypes.c_char_p, ctypes.POINTER(ci)]
     L.ddfmm_stage_times.argtypes = [vp, ci, ctypes.c_char_p, ctypes.POINTER(cd), ctypes.POINTER(cd),
                                      ctypes.POINTER(ci), ctypes.POINTER(ci)]
     L.ddfmm_stage_exec.argtypes = [vp, ci, ctypes.POINTER(cd), ctypes.POINTER(ci)]

@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libddfmm.so")
-SOURCES = ["tree.cpp", "mlib.cpp", "planner.cpp", "capi.cpp", "kernels.hip"]
+SOURCES = ["tree.cpp", "mlib.cpp", "planner.cpp", "tiles.cpp", "capi.cpp", "kernels.hip"]
 
 
 def hipcc() -> str:
@@ -33,6 +33,9 @@ def build(verbose: bool = False, jobs: int = 5) -> str:
               # MFMA accumulators in ordinary VGPRs: with the AGPR form the GEMM's
               # loop-carried accumulators are copied between register files every slab
               "-mllvm", "-amdgpu-mfma-vgpr-form=1",
+              # only the C ABI of include/ddfmm.h is exported: callers that include the
+              # C++ drop-in headers see none of the library's internal C++ symbols
+              "-fvisibility=hidden", "-fvisibility-inlines-hidden",
               "-Wno-unused-result", "-I", CSRC]
     procs, objs = [], []
     for s in SOURCES:

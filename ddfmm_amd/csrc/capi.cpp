@@ -2,10 +2,13 @@
 // translation-matrix generation) and evaluation (stage sequence on one HIP stream).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -33,13 +36,40 @@ struct HipFail : std::runtime_error {
         if (e_ != hipSuccess)                                                                 \
             throw HipFail(std::string(#expr) + ": " + hipGetErrorString(e_));                 \
     } while (0)
+// Parsed operator libraries shared by every context that uses them: a handle holds a
+// reference, and ddfmm_setup reuses a library that a live handle (or a concurrent setup)
+// still holds instead of reading the files again.
+std::mutex g_mlib_mu;
+std::map<std::pair<std::string, std::string>, std::weak_ptr<const Mlib>> g_mlib_cache;
+
+std::shared_ptr<const Mlib> load_mlib(const std::string& ld, const std::string& hd) {
+    std::lock_guard<std::mutex> lock(g_mlib_mu);
+    auto key = std::make_pair(ld, hd);
+    auto it = g_mlib_cache.find(key);
+    if (it != g_mlib_cache.end())
+        if (auto sp = it->second.lock()) return sp;
+    auto m = std::make_shared<Mlib>();
+    m->load(ld, hd);
+    std::shared_ptr<const Mlib> c = m;
+    g_mlib_cache[key] = c;
+    return c;
+}
 }  // namespace
+
+struct ddfmm_mlib {
+    std::shared_ptr<const Mlib> lib;
+};
 
 struct ddfmm_ctx {
     int device = -1;
     hipStream_t stream = nullptr;
     hipStream_t aux = nullptr;                 // second stream of an evaluation (near field, LF lists)
     hipEvent_t sync_ev[STAGE_EVENTS] = {};     // fork/join events between the two streams
+    hipEvent_t done_ev = nullptr;              // end of the last evaluation (on its stream)
+    bool done_recorded = false;
+    bool use_graph = false;                    // replay a captured graph of the stage sequence
+    std::map<int, hipGraphExec_t> graphs;      // one per right-hand-side count
+    size_t tiles_bytes = 0;                    // device bytes of the GEMM tile table
     Params prm;
     int nrhs_max = 1, precision = 0;
     HostPlan hp;
@@ -66,7 +96,12 @@ struct ddfmm_ctx {
         if (p) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
         return p;
     }
+    void drop_graphs() {
+        for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+        graphs.clear();
+    }
     void release() {
+        drop_graphs();
         for (void* p : allocs) (void)hipFree(p);
         allocs.clear();
         for (auto e : ev) (void)hipEventDestroy(e);
@@ -90,6 +125,7 @@ int ddfmm_create(int device, ddfmm_ctx** out) {
             HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
             HIPCHK(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
             for (auto& e : c->sync_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&c->done_ev, hipEventDisableTiming));
         } catch (const std::exception& e) {
             delete c;
             return fail(DDFMM_EDEVICE, e.what());
@@ -108,6 +144,7 @@ int ddfmm_destroy(ddfmm_ctx* c) {
         c->release();
         for (auto e : c->sync_ev)
             if (e) (void)hipEventDestroy(e);
+        if (c->done_ev) (void)hipEventDestroy(c->done_ev);
         if (c->aux) (void)hipStreamDestroy(c->aux);
         if (c->stream) (void)hipStreamDestroy(c->stream);
     }
@@ -115,9 +152,57 @@ int ddfmm_destroy(ddfmm_ctx* c) {
     return DDFMM_OK;
 }
 
+static int setup_common(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_params* p,
+                        std::shared_ptr<const Mlib> shared, const char* ldfile, const char* hdfile);
+
 int ddfmm_setup(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_params* p,
                 const char* ldfile, const char* hdfile) {
     if (!c || !pos || !p || !ldfile || !hdfile) return fail(DDFMM_EINVAL, "null argument");
+    return setup_common(c, pos, n, p, nullptr, ldfile, hdfile);
+}
+
+int ddfmm_setup_mlib(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_params* p, const ddfmm_mlib* m) {
+    if (!c || !pos || !p || !m || !m->lib) return fail(DDFMM_EINVAL, "null argument");
+    return setup_common(c, pos, n, p, m->lib, nullptr, nullptr);
+}
+
+int ddfmm_mlib_load(const char* ldfile, const char* hdfile, ddfmm_mlib** out) {
+    if (!ldfile || !hdfile || !out) return fail(DDFMM_EINVAL, "null argument");
+    try {
+        auto* m = new ddfmm_mlib();
+        try {
+            m->lib = load_mlib(ldfile, hdfile);
+        } catch (...) {
+            delete m;
+            throw;
+        }
+        *out = m;
+    } catch (const std::exception& e) {
+        return fail(DDFMM_ESETUP, e.what());
+    }
+    return DDFMM_OK;
+}
+
+int ddfmm_mlib_free(ddfmm_mlib* m) {
+    delete m;
+    return DDFMM_OK;
+}
+
+int ddfmm_mlib_widths(const ddfmm_mlib* m, int cap, double* ld_widths, int* nld, double* hd_widths, int* nhd) {
+    if (!m || !m->lib || !nld || !nhd) return fail(DDFMM_EINVAL, "null argument");
+    *nld = (int)m->lib->ld.size();
+    *nhd = (int)m->lib->hd.size();
+    int i = 0;
+    for (auto& kv : m->lib->ld)
+        if (ld_widths && i < cap) ld_widths[i++] = kv.first;
+    i = 0;
+    for (auto& kv : m->lib->hd)
+        if (hd_widths && i < cap) hd_widths[i++] = kv.first;
+    return DDFMM_OK;
+}
+
+static int setup_common(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_params* p,
+                        std::shared_ptr<const Mlib> shared, const char* ldfile, const char* hdfile) {
     if (n <= 0 || n > (int64_t)INT32_MAX) return fail(DDFMM_EINVAL, "point count out of range");
     if (p->kernel != 0 && p->kernel != 1) return fail(DDFMM_EINVAL, "Unknown kernel type " + std::to_string(p->kernel));
     if (p->ACCU < 1 || p->ACCU > 3) return fail(DDFMM_EINVAL, "ACCU must be 1, 2 or 3");
@@ -143,9 +228,8 @@ int ddfmm_setup(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_params* 
         q.nrhs_max = p->nrhs_max;
         c->nrhs_max = p->nrhs_max;
         c->precision = p->precision;
-        Mlib mlib;
-        mlib.load(ldfile, hdfile);
-        build_plan(q, pos, n, mlib, c->hp, c->device < 0);
+        std::shared_ptr<const Mlib> mlib = shared ? shared : load_mlib(ldfile, hdfile);
+        build_plan(q, pos, n, *mlib, c->hp, c->device < 0);
     } catch (const HipFail& e) {
         return fail(DDFMM_EDEVICE, e.what());
     } catch (const std::exception& e) {
@@ -170,6 +254,7 @@ int ddfmm_setup(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_params* 
             HIPCHK(hipMemcpy(d.opmat, h.opmat.data(), h.opmat.size() * sizeof(double2), hipMemcpyHostToDevice));
         d.ktasks = c->upload(h.ktasks); d.ksrcs = c->upload(h.ksrcs);
         d.groups = c->upload(h.groups); d.tiles = c->upload(h.tiles);
+        c->tiles_bytes = h.tiles.size() * sizeof(GTile);
         d.colB = c->upload(h.colB); d.colC = c->upload(h.colC); d.colA = c->upload(h.colA);
         d.fsrc = c->upload(h.fsrc); d.vtrg = c->upload(h.vtrg);
         d.vchunk = c->upload(h.vchunk); d.vsrc = c->upload(h.vsrc); d.vpk = c->upload(h.vpk);
@@ -181,6 +266,7 @@ int ddfmm_setup(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_params* 
         }
         d.den_io = c->alloc<double2>(h.N * c->nrhs_max);
         d.val_io = c->alloc<double2>(h.N * c->nrhs_max);
+        d.io = c->alloc<IoPtrs>(1);
         // translation matrices generated on the device (temporary generator tables)
         if (!h.kgen.empty()) {
             double* gp = nullptr;
@@ -225,15 +311,11 @@ int ddfmm_setup(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_params* 
 // makes the second stream wait for the work issued so far on `st`, ST_JOIN the reverse,
 // so that the caller's stream sees a complete evaluation.  With timing on, each stage is
 // bracketed by events on its own stream.
-static int run_stages(ddfmm_ctx* c, int nrhs, hipStream_t st) {
+static void issue_stages(ddfmm_ctx* c, int nrhs, hipStream_t st, bool timing, bool serial = false) {
     const auto& stages = c->hp.stages;
-    if (c->timing && c->ev.size() != 2 * stages.size()) {
-        for (auto e : c->ev) (void)hipEventDestroy(e);
-        c->ev.assign(2 * stages.size(), nullptr);
-        for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
-    }
     for (size_t i = 0; i < stages.size(); ++i) {
         const Stage& s = stages[i];
+        if (serial && (s.kind == ST_FORK || s.kind == ST_JOIN)) continue;
         if (s.kind == ST_FORK) {
             HIPCHK(hipEventRecord(c->sync_ev[s.buf], st));
             HIPCHK(hipStreamWaitEvent(c->aux, c->sync_ev[s.buf], 0));
@@ -244,12 +326,58 @@ static int run_stages(ddfmm_ctx* c, int nrhs, hipStream_t st) {
             HIPCHK(hipStreamWaitEvent(st, c->sync_ev[s.buf], 0));
             continue;
         }
-        hipStream_t ss = s.stream ? c->aux : st;
-        if (c->timing) HIPCHK(hipEventRecord(c->ev[2 * i], ss));
+        hipStream_t ss = (s.stream && !serial) ? c->aux : st;
+        if (timing) HIPCHK(hipEventRecord(c->ev[2 * i], ss));
         hipError_t e = launch_stage(c->dp, s, nrhs, ss);
         if (e != hipSuccess) throw HipFail(std::string("stage ") + s.name + ": " + hipGetErrorString(e));
-        if (c->timing) HIPCHK(hipEventRecord(c->ev[2 * i + 1], ss));
+        if (timing) HIPCHK(hipEventRecord(c->ev[2 * i + 1], ss));
     }
+}
+
+// The stage sequence for nrhs captured once from the context's stream (the second stream
+// joins the capture through the fork events) and instantiated; replays read the caller's
+// buffers through DevPlan::io.
+static hipGraphExec_t graph_for(ddfmm_ctx* c, int nrhs) {
+    auto it = c->graphs.find(nrhs);
+    if (it != c->graphs.end()) return it->second;
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    try {
+        issue_stages(c, nrhs, c->stream, false);
+    } catch (...) {
+        (void)hipStreamEndCapture(c->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    HIPCHK(hipStreamEndCapture(c->stream, &g));
+    hipGraphExec_t x = nullptr;
+    hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) throw HipFail(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    c->graphs[nrhs] = x;
+    return x;
+}
+
+// One evaluation on `st` reading den and writing val (device buffers).  Evaluations of
+// one context share its work buffers, so each one first waits for the end of the
+// previous one (which may have been issued on another stream).
+static int run_eval(ddfmm_ctx* c, const double2* den, double2* val, int nrhs, hipStream_t st) {
+    const auto& stages = c->hp.stages;
+    if (c->done_recorded) HIPCHK(hipStreamWaitEvent(st, c->done_ev, 0));
+    HIPCHK(launch_set_io(c->dp.io, den, val, st));
+    if (c->use_graph && !c->timing) {
+        hipGraphExec_t x = graph_for(c, nrhs);
+        HIPCHK(hipGraphLaunch(x, st));
+    } else {
+        if (c->timing && c->ev.size() != 2 * stages.size()) {
+            for (auto e : c->ev) (void)hipEventDestroy(e);
+            c->ev.assign(2 * stages.size(), nullptr);
+            for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
+        }
+        issue_stages(c, nrhs, st, c->timing);
+    }
+    HIPCHK(hipEventRecord(c->done_ev, st));
+    c->done_recorded = true;
     if (c->timing) {
         HIPCHK(hipStreamSynchronize(st));
         c->stage_ms.assign(stages.size(), 0.0);
@@ -270,8 +398,9 @@ int ddfmm_eval(ddfmm_ctx* c, const double* den, double* val, int nrhs) {
     try {
         HIPCHK(hipSetDevice(c->device));
         size_t bytes = (size_t)c->hp.N * nrhs * sizeof(double2);
+        if (c->done_recorded) HIPCHK(hipStreamWaitEvent(c->stream, c->done_ev, 0));
         HIPCHK(hipMemcpyAsync(c->dp.den_io, den, bytes, hipMemcpyHostToDevice, c->stream));
-        run_stages(c, nrhs, c->stream);
+        run_eval(c, c->dp.den_io, c->dp.val_io, nrhs, c->stream);
         HIPCHK(hipMemcpyAsync(val, c->dp.val_io, bytes, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
     } catch (const std::exception& e) {
@@ -285,20 +414,146 @@ int ddfmm_eval_device(ddfmm_ctx* c, const void* den_dev, void* val_dev, int nrhs
     if (!c->ready || c->device < 0) return fail(DDFMM_ESTATE, "context not set up on a device");
     if (nrhs < 1 || nrhs > c->nrhs_max) return fail(DDFMM_EINVAL, "nrhs out of range [1, nrhs_max]");
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
-    double2* sd = c->dp.den_io;
-    double2* sv = c->dp.val_io;
-    c->dp.den_io = (double2*)den_dev;
-    c->dp.val_io = (double2*)val_dev;
-    int rc = DDFMM_OK;
     try {
         HIPCHK(hipSetDevice(c->device));
-        run_stages(c, nrhs, st);
+        run_eval(c, (const double2*)den_dev, (double2*)val_dev, nrhs, st);
     } catch (const std::exception& e) {
-        rc = fail(DDFMM_EDEVICE, e.what());
+        return fail(DDFMM_EDEVICE, e.what());
     }
-    c->dp.den_io = sd;
-    c->dp.val_io = sv;
-    return rc;
+    return DDFMM_OK;
+}
+
+int ddfmm_set_graph(ddfmm_ctx* c, int on) {
+    if (!c) return fail(DDFMM_EINVAL, "null argument");
+    if (!on && c->device >= 0) {
+        (void)hipSetDevice(c->device);
+        if (c->stream) (void)hipStreamSynchronize(c->stream);
+        c->drop_graphs();
+    }
+    c->use_graph = on != 0;
+    return DDFMM_OK;
+}
+
+// ---------------------------------------------------------------- GEMM tile autotuning
+// Upload the host tiles after a rebuild: the device table is replaced (the context's
+// stream is idle), the captured graphs are dropped and the stage times reset.
+static void upload_tiles(ddfmm_ctx* c) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipStreamSynchronize(c->aux));
+    auto it = std::find(c->allocs.begin(), c->allocs.end(), (void*)c->dp.tiles);
+    if (it != c->allocs.end()) {
+        HIPCHK(hipFree(*it));
+        c->allocs.erase(it);
+        c->device_bytes -= (int64_t)c->tiles_bytes;
+    }
+    c->dp.tiles = c->upload(c->hp.tiles);
+    c->tiles_bytes = c->hp.tiles.size() * sizeof(GTile);
+    c->drop_graphs();
+    c->ev.clear();
+    c->stage_ms.clear();
+}
+
+// Per logical GEMM stage, the milliseconds of its launches in a stage-by-stage evaluation
+// on one stream (no overlap with the second stream's work), best of reps.
+static std::vector<double> time_logical(ddfmm_ctx* c, int nrhs, int reps) {
+    const auto& stages = c->hp.stages;
+    std::vector<double> best(c->hp.lgemm.size(), 1e30);
+    if (c->ev.size() != 2 * stages.size()) {
+        for (auto e : c->ev) (void)hipEventDestroy(e);
+        c->ev.assign(2 * stages.size(), nullptr);
+        for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
+    }
+    HIPCHK(launch_set_io(c->dp.io, c->dp.den_io, c->dp.val_io, c->stream));
+    for (int r = 0; r <= reps; ++r) {            // the first run warms up
+        issue_stages(c, nrhs, c->stream, true, true);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (r == 0) continue;
+        std::vector<double> t(c->hp.lgemm.size(), 0.0);
+        for (size_t i = 0; i < stages.size(); ++i) {
+            if (stages[i].kind != ST_GEMM) continue;
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, c->ev[2 * i], c->ev[2 * i + 1]));
+            t[stages[i].lstage] += ms;
+        }
+        for (size_t l = 0; l < t.size(); ++l) best[l] = std::min(best[l], t[l]);
+    }
+    return best;
+}
+
+int ddfmm_autotune(ddfmm_ctx* c, int nrhs, int reps, int cap, int* mbmax, int* count, double* ms_before,
+                   double* ms_after) {
+    if (!c || !count) return fail(DDFMM_EINVAL, "null argument");
+    if (!c->ready || c->device < 0) return fail(DDFMM_ESTATE, "context not set up on a device");
+    if (nrhs < 1 || nrhs > c->nrhs_max) return fail(DDFMM_EINVAL, "nrhs out of range [1, nrhs_max]");
+    reps = std::max(1, reps);
+    const int nl = (int)c->hp.lgemm.size();
+    *count = nl;
+    try {
+        HIPCHK(hipSetDevice(c->device));
+        if (c->done_recorded) HIPCHK(hipStreamWaitEvent(c->stream, c->done_ev, 0));
+        HIPCHK(hipMemsetAsync(c->dp.den_io, 0, (size_t)c->hp.N * nrhs * sizeof(double2), c->stream));
+        std::vector<int> cur(nl);
+        for (int l = 0; l < nl; ++l) cur[l] = c->hp.lgemm[l].mbmax;
+        // the planner's tiling (height limit GEMM_MB, rows split further for small stages)
+        std::vector<double> t0 = time_logical(c, nrhs, reps);
+        std::vector<int> best = cur;
+        std::vector<double> bt = t0;
+        for (int m = 1; m <= GEMM_MB; ++m) {
+            retile(c->hp, std::vector<int>(nl, m));
+            upload_tiles(c);
+            std::vector<double> t = time_logical(c, nrhs, reps);
+            for (int l = 0; l < nl; ++l)
+                if (t[l] < bt[l]) {
+                    bt[l] = t[l];
+                    best[l] = m;
+                }
+        }
+        retile(c->hp, best);
+        upload_tiles(c);
+        std::vector<double> t1 = time_logical(c, nrhs, reps);
+        double a = 0, b = 0;
+        for (int l = 0; l < nl; ++l) {
+            a += t0[l];
+            b += t1[l];
+            if (mbmax && l < cap) mbmax[l] = best[l];
+        }
+        if (ms_before) *ms_before = a;
+        if (ms_after) *ms_after = b;
+    } catch (const std::exception& e) {
+        return fail(DDFMM_EDEVICE, e.what());
+    }
+    return DDFMM_OK;
+}
+
+int ddfmm_set_tiles(ddfmm_ctx* c, int count, const int* mbmax) {
+    if (!c || !mbmax) return fail(DDFMM_EINVAL, "null argument");
+    if (!c->ready) return fail(DDFMM_ESTATE, "context not set up");
+    if (count != (int)c->hp.lgemm.size())
+        return fail(DDFMM_EINVAL, "tile table has " + std::to_string(count) + " entries, the plan has " +
+                                      std::to_string(c->hp.lgemm.size()) + " GEMM stages");
+    try {
+        retile(c->hp, std::vector<int>(mbmax, mbmax + count));
+        if (c->device >= 0) {
+            HIPCHK(hipSetDevice(c->device));
+            upload_tiles(c);
+        }
+    } catch (const HipFail& e) {
+        return fail(DDFMM_EDEVICE, e.what());
+    } catch (const std::exception& e) {
+        return fail(DDFMM_EINVAL, e.what());
+    }
+    return DDFMM_OK;
+}
+
+int ddfmm_get_tiles(ddfmm_ctx* c, int cap, int* mbmax, char* names /* cap x 24 */, int* count) {
+    if (!c || !count) return fail(DDFMM_EINVAL, "null argument");
+    if (!c->ready) return fail(DDFMM_ESTATE, "context not set up");
+    *count = (int)c->hp.lgemm.size();
+    for (int l = 0; l < *count && l < cap; ++l) {
+        if (mbmax) mbmax[l] = c->hp.lgemm[l].mbmax;
+        if (names) std::memcpy(names + 24 * l, c->hp.lgemm[l].name, 24);
+    }
+    return DDFMM_OK;
 }
 
 int ddfmm_direct_sum(int kernel, const double* trg, int m, const double* src, int n, const double* den, int nrhs,
@@ -307,24 +562,26 @@ int ddfmm_direct_sum(int kernel, const double* trg, int m, const double* src, in
     if (kernel != 0 && kernel != 1) return fail(DDFMM_EINVAL, "Unknown kernel type " + std::to_string(kernel));
     if (m == 0) return DDFMM_OK;
     double *dt = nullptr, *ds = nullptr;
-    double2 *dd = nullptr, *dout = nullptr;
+    double2 *dd = nullptr, *dout = nullptr, *part = nullptr;
     int rc = DDFMM_OK;
     try {
+        const DirectSplit sp = direct_split(m, n, nrhs);
         HIPCHK(hipMalloc(&dt, (size_t)m * 3 * sizeof(double)));
         HIPCHK(hipMalloc(&ds, (size_t)std::max(n, 1) * 3 * sizeof(double)));
         HIPCHK(hipMalloc(&dd, (size_t)std::max(n, 1) * nrhs * sizeof(double2)));
         HIPCHK(hipMalloc(&dout, (size_t)m * nrhs * sizeof(double2)));
+        if (sp.part_elems > 0) HIPCHK(hipMalloc(&part, (size_t)sp.part_elems * sizeof(double2)));
         HIPCHK(hipMemcpy(dt, trg, (size_t)m * 3 * sizeof(double), hipMemcpyHostToDevice));
         if (n > 0) {
             HIPCHK(hipMemcpy(ds, src, (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(dd, den, (size_t)n * nrhs * sizeof(double2), hipMemcpyHostToDevice));
         }
-        HIPCHK(launch_direct(dt, m, ds, n, dd, nrhs, dout, kernel, nullptr));
+        HIPCHK(launch_direct(dt, m, ds, n, dd, nrhs, dout, part, kernel, nullptr));
         HIPCHK(hipMemcpy(out, dout, (size_t)m * nrhs * sizeof(double2), hipMemcpyDeviceToHost));
     } catch (const std::exception& e) {
         rc = fail(DDFMM_EDEVICE, e.what());
     }
-    (void)hipFree(dt); (void)hipFree(ds); (void)hipFree(dd); (void)hipFree(dout);
+    (void)hipFree(dt); (void)hipFree(ds); (void)hipFree(dd); (void)hipFree(dout); (void)hipFree(part);
     return rc;
 }
 

@@ -13,6 +13,14 @@ struct BufTable {
     int64_t stride[BUF_COUNT + 1]; // elements per rhs slab
 };
 
+// The caller's density and potential buffers of the current evaluation, read by the
+// gather and scatter kernels from device memory, so that a captured graph of the stage
+// sequence serves every call (set before each evaluation by launch_set_io).
+struct IoPtrs {
+    const double2* den;            // [rhs][N] input order (gather source)
+    double2* val;                  // [rhs][N] input order (scatter target)
+};
+
 struct DevPlan {
     int64_t N = 0;
     int P = 0, NL = 0, ktype = 0;
@@ -34,15 +42,27 @@ struct DevPlan {
     int* vpk = nullptr;
     RTarget* rtrg = nullptr;
     BufTable bt{};
-    double2* den_io = nullptr;     // [rhs][N] input order (gather source)
-    double2* val_io = nullptr;     // [rhs][N] input order (scatter target)
+    double2* den_io = nullptr;     // context-owned [rhs][N] staging of the host-buffer entry point
+    double2* val_io = nullptr;
+    IoPtrs* io = nullptr;          // device copy of the current evaluation's buffers
 };
 
 hipError_t launch_stage(const DevPlan& d, const Stage& s, int nrhs, hipStream_t st);
+hipError_t launch_set_io(IoPtrs* io, const double2* den, double2* val, hipStream_t st);
 hipError_t launch_setup_gemm(const DevPlan& d, const Stage& s, hipStream_t st);   // FP64, on the operator table
 hipError_t launch_kgen(const KGen* jobs, int njobs, const double* gp, double2* opmat, int ktype, hipStream_t st);
+// Direct sum split over S source ranges (chunk sources each, RC right-hand sides per
+// block); with S > 1 the ranges write part (S x nrhs x M complex, part_elems) and a
+// fixed-order reduction adds them into out.  The split depends on (M, Ns, nrhs) only, so
+// every entry point that uses it sums in the same order.
+struct DirectSplit {
+    int S = 1, chunk = 0, rc = 1;
+    int64_t part_elems = 0;
+};
+DirectSplit direct_split(int M, int Ns, int nrhs);
 hipError_t launch_direct(const double* trg, int M, const double* src, int Ns, const double2* den, int nrhs,
-                         double2* out, int ktype, hipStream_t st);
+                         double2* out, double2* part, int ktype, hipStream_t st);
+
 hipError_t launch_kmatrix(const double* trg, int M, const double* src, int Ns, void* out, int ktype, int f32,
                           hipStream_t st);
 hipError_t launch_to_f32(const double2* in, float2* out, int64_t n, hipStream_t st);

@@ -558,7 +558,7 @@ __device__ __forceinline__ void dft12_line(const V* in, int is, V* out, int os) 
 
 template <class V>
 __global__ __launch_bounds__(256) void fft_fwd_kernel(const FftSrc* __restrict__ fs, const int* __restrict__ grid,
-                                                      int NL, BufTable bt, int f0, int nf) {
+                                                      int NL, BufTable bt, int f0, int nf, int nrhs) {
     __shared__ V sub[216], A[432], B[864];
     const int rhs = blockIdx.y;
     const int t = threadIdx.x;
@@ -566,7 +566,8 @@ __global__ __launch_bounds__(256) void fft_fwd_kernel(const FftSrc* __restrict__
     const FftSrc f = fs[f0 + xcd_item(bi, nf)];
     __syncthreads();
     const V* in = bufp<V>(bt, BUF_LF_UPEQ) + (int64_t)rhs * bt.stride[BUF_LF_UPEQ] + f.in_off;
-    V* out = bufp<V>(bt, BUF_SPEC) + (int64_t)rhs * bt.stride[BUF_SPEC] + f.spec_off;
+    // spectra right-hand side fastest: frequency k of this source at (spec_off + k) * nrhs + rhs
+    V* out = bufp<V>(bt, BUF_SPEC) + f.spec_off * nrhs + rhs;
     if (t < 216) sub[t] = mk<V>(0, 0);
     __syncthreads();
     for (int k = t; k < NL; k += blockDim.x) {
@@ -581,7 +582,7 @@ __global__ __launch_bounds__(256) void fft_fwd_kernel(const FftSrc* __restrict__
         dft12_line<V, 6, 12, 6, 0, -1>(A + (t % 12) + 72 * (t / 12), 12, B + (t % 12) + 144 * (t / 12), 12);
     __syncthreads();
     if (t < 144)         // z lines (kx, ky): B -> spectrum [kz][ky][kx]
-        dft12_line<V, 6, 12, 6, 0, -1>(B + t, 144, out + t, 144);
+        dft12_line<V, 6, 12, 6, 0, -1>(B + t, 144, out + (int64_t)t * nrhs, 144 * nrhs);
     }
 }
 
@@ -689,6 +690,124 @@ __global__ __launch_bounds__(256) void vlist_acc_kernel(const VTarget* __restric
     }
 }
 
+// V list, accumulation for several right-hand sides (nrhs > 1).  The source spectra are
+// stored right-hand side fastest, SPEC[(row * (2P)^3 + f) * nrhs + r], so the values of
+// one source for F frequencies and RC = R1 * R2 right-hand sides of a block are one
+// 128-byte run per frequency.  A block owns (chunk, slice of F frequencies, group of RC
+// right-hand sides) and stages the slice of the translation tensor (343 offsets, shared
+// by the group) and the union's source rows (128 bytes each).  Eight lanes work on one
+// target: lane l takes frequency l / R1 and the R2 right-hand sides (l % R1) * R2 + q,
+// so the eight lanes read one whole source row per pair (all 32 banks, conflict-free)
+// and one tensor value serves R2 complex FMAs.  The accumulated target spectra go to
+// TSPEC in the [rhs][target][(2P)^3] layout of the inverse transform.
+template <class V> struct VMulti { static constexpr int R2 = 16 / (int)sizeof(V); };   // float2: 2, double2: 1
+
+template <class V, int F, int R1>
+__global__ __launch_bounds__(256) void vlist_acc_multi_kernel(const VTarget* __restrict__ vt,
+                                                              const VChunk* __restrict__ vc,
+                                                              const int* __restrict__ vsrc,
+                                                              const int* __restrict__ vpk,
+                                                              const V* __restrict__ opm, BufTable bt, int v0,
+                                                              int c0, int nchunk, int nrhs) {
+    constexpr int G3 = 1728, R2 = VMulti<V>::R2, RC = R1 * R2, ROW = F * RC, NSL = G3 / F;
+    static_assert(F * R1 == 8, "eight lanes per target");
+    static_assert(ROW * (int)sizeof(V) == 128, "128-byte source rows");
+    __shared__ V tab[343 * F];
+    __shared__ __align__(16) V src[VCH_SRC * ROW];
+    const int t = threadIdx.x, l = t & 7;
+    const int fl = l / R1, rq = (l % R1) * R2;
+    const int item = xcd_item(blockIdx.x, NSL * nchunk);
+    const int f0 = (item / nchunk) * F;
+    const int rg = blockIdx.y * RC;
+    const VChunk ch = vc[c0 + item % nchunk];
+    const V* T0 = opm + vt[v0].tab_off + f0;
+    const V* spec = bufp<V>(bt, BUF_SPEC);
+    {   // staging, all loads of a thread issued before its LDS writes; row 0 is zero
+        constexpr int NT = (343 * F + 255) / 256, NS = (VCH_SRC * ROW + 255) / 256;
+        const int nu = (ch.u1 - ch.u0) * ROW;
+        V a[NT], b[NS];
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+            const int e = t + 256 * k;
+            a[k] = e < 343 * F ? T0[(int64_t)(e / F) * G3 + e % F] : mk<V>(0, 0);
+        }
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int e = t + 256 * k;
+            const int u = e / ROW, w = e % ROW, f = w / RC, r = rg + w % RC;
+            V v = mk<V>(0, 0);
+            if (e >= ROW && e < nu && r < nrhs)
+                v = spec[((int64_t)vsrc[ch.u0 + u] * G3 + f0 + f) * nrhs + r];
+            b[k] = v;
+        }
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+            const int e = t + 256 * k;
+            if (e < 343 * F) tab[e] = a[k];
+        }
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int e = t + 256 * k;
+            if (e < nu) src[e] = b[k];
+        }
+    }
+    __syncthreads();
+    const V* tabl = tab + fl;
+    const V* srcl = src + l * R2;
+    V* out = bufp<V>(bt, BUF_TSPEC);
+    const int64_t os = bt.stride[BUF_TSPEC];
+    for (int i = ch.t0 + t / 8; i < ch.t1; i += 32) {
+        const VTarget tg = vt[i];
+        V acc[R2];
+#pragma unroll
+        for (int q = 0; q < R2; ++q) acc[q] = mk<V>(0, 0);
+        const int n = (tg.p1 - tg.p0 + 7) & ~7;
+        const int4* pk = (const int4*)(vpk + tg.k0);
+        auto step = [&](const int4* q4) {
+            V tv[8], sv[8][R2];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int4 q = q4[u / 4];
+                const int w = (u % 4 == 0) ? q.x : (u % 4 == 1) ? q.y : (u % 4 == 2) ? q.z : q.w;
+                tv[u] = tabl[(w >> 16) * F];
+                const V* sp = srcl + (w & 0xffff) * ROW;
+                if constexpr (R2 == 2) {
+                    const float4 x = *(const float4*)sp;
+                    sv[u][0] = mk<V>(x.x, x.y);
+                    sv[u][R2 - 1] = mk<V>(x.z, x.w);
+                } else {
+                    sv[u][0] = sp[0];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+#pragma unroll
+                for (int q = 0; q < R2; ++q) cfma(acc[q], sv[u][q], tv[u]);   // operand order of the 1-RHS kernel
+        };
+        int4 qa[2], qb[2];
+        qa[0] = pk[0];
+        qa[1] = pk[1];
+        for (int j = 0; j < n; j += 16) {      // two steps of 8 pairs per pass, the next 8 in flight
+            if (j + 8 < n) {
+                qb[0] = pk[(j + 8) / 4];
+                qb[1] = pk[(j + 8) / 4 + 1];
+            }
+            step(qa);
+            if (j + 8 >= n) break;
+            if (j + 16 < n) {
+                qa[0] = pk[(j + 16) / 4];
+                qa[1] = pk[(j + 16) / 4 + 1];
+            }
+            step(qb);
+        }
+#pragma unroll
+        for (int q = 0; q < R2; ++q) {
+            const int r = rg + rq + q;
+            if (r < nrhs) out[(int64_t)r * os + (int64_t)(i - v0) * G3 + f0 + fl] = acc[q];
+        }
+    }
+}
+
 // V list, inverse: one block per target, the accumulated spectrum back to the 6^3 sub-cube
 // (pruned inverse DFT, 1/(2P)^3 scaling) and added into the target's check values.
 template <class V>
@@ -755,8 +874,9 @@ __global__ void zero_kernel(V* p, int64_t n) {
 }
 
 template <class V>
-__global__ void gather_kernel(const double2* __restrict__ in, V* __restrict__ out, const int* __restrict__ perm,
+__global__ void gather_kernel(const IoPtrs* __restrict__ io, V* __restrict__ out, const int* __restrict__ perm,
                               int64_t N, int nrhs) {
+    const double2* __restrict__ in = io->den;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < N * nrhs; e += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = e / N, i = e % N;
         const double2 v = in[r * N + perm[i]];
@@ -765,8 +885,9 @@ __global__ void gather_kernel(const double2* __restrict__ in, V* __restrict__ ou
 }
 
 template <class V>
-__global__ void scatter_kernel(const V* __restrict__ in, double2* __restrict__ out, const int* __restrict__ perm,
+__global__ void scatter_kernel(const V* __restrict__ in, const IoPtrs* __restrict__ io, const int* __restrict__ perm,
                                int64_t N, int nrhs) {
+    double2* __restrict__ out = io->val;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < N * nrhs; e += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = e / N, i = e % N;
         const V v = in[e];
@@ -774,34 +895,66 @@ __global__ void scatter_kernel(const V* __restrict__ in, double2* __restrict__ o
     }
 }
 
-// direct sum u(t_i) = sum_j G(t_i, s_j) f_j (Wave3d::check, wave3d_check.cpp:66-76)
-__global__ __launch_bounds__(256) void direct_kernel(const double* __restrict__ trg, int M, const double* __restrict__ src,
-                                                     int Ns, const double2* __restrict__ den, int nrhs,
-                                                     double2* __restrict__ out, int ktype) {
+// direct sum u(t_i) = sum_j G(t_i, s_j) f_j (Wave3d::check, wave3d_check.cpp:66-76).
+// Block (target tile x, source range y, right-hand-side group z): 256 targets, one thread
+// each, over the sources [y * chunk, (y + 1) * chunk) staged through LDS in tiles of 256;
+// G is evaluated once per pair for the RC right-hand sides of the group.  With one source
+// range the sums go straight to out, otherwise to part[y][rhs][target] for the fixed-order
+// reduction below.
+template <int RC>
+__global__ __launch_bounds__(256) void direct_kernel(const double* __restrict__ trg, int M,
+                                                     const double* __restrict__ src, int Ns,
+                                                     const double2* __restrict__ den, int nrhs,
+                                                     double2* __restrict__ out, int chunk, int ktype) {
     __shared__ double s3[256 * 3];
-    __shared__ double2 sd[256];
+    __shared__ double2 sd[RC][256];
     const int i = blockIdx.x * 256 + threadIdx.x;
-    const int rhs = blockIdx.y;
+    const int r0 = blockIdx.z * RC, nr = min(RC, nrhs - r0);
     const bool ok = i < M;
     double tx = 0, ty = 0, tz = 0;
     if (ok) { tx = trg[3 * i]; ty = trg[3 * i + 1]; tz = trg[3 * i + 2]; }
-    double2 acc = make_double2(0, 0);
-    for (int sb = 0; sb < Ns; sb += 256) {
-        const int n = min(256, Ns - sb);
+    double2 acc[RC];
+#pragma unroll
+    for (int q = 0; q < RC; ++q) acc[q] = make_double2(0, 0);
+    const int j0 = blockIdx.y * chunk, j1 = min(Ns, j0 + chunk);
+    for (int sb = j0; sb < j1; sb += 256) {
+        const int n = min(256, j1 - sb);
         __syncthreads();
         if ((int)threadIdx.x < n) {
             const int j = sb + threadIdx.x;
             s3[3 * threadIdx.x] = src[3 * j];
             s3[3 * threadIdx.x + 1] = src[3 * j + 1];
             s3[3 * threadIdx.x + 2] = src[3 * j + 2];
-            sd[threadIdx.x] = den[(int64_t)rhs * Ns + j];
+#pragma unroll
+            for (int q = 0; q < RC; ++q)
+                sd[q][threadIdx.x] = q < nr ? den[(int64_t)(r0 + q) * Ns + j] : make_double2(0, 0);
         }
         __syncthreads();
         if (ok)
-            for (int j = 0; j < n; ++j)
-                cfma(acc, green64(tx - s3[3 * j], ty - s3[3 * j + 1], tz - s3[3 * j + 2], ktype), sd[j]);
+            for (int j = 0; j < n; ++j) {
+                const double2 g = green64(tx - s3[3 * j], ty - s3[3 * j + 1], tz - s3[3 * j + 2], ktype);
+#pragma unroll
+                for (int q = 0; q < RC; ++q) cfma(acc[q], g, sd[q][j]);
+            }
     }
-    if (ok) out[(int64_t)rhs * M + i] = acc;
+    if (!ok) return;
+    double2* o = out + (int64_t)blockIdx.y * nrhs * M + i;
+#pragma unroll
+    for (int q = 0; q < RC; ++q)
+        if (q < nr) o[(int64_t)(r0 + q) * M] = acc[q];
+}
+
+// out[e] = sum_{y < S} part[y][e] in the order y = 0, 1, ... (deterministic)
+__global__ void direct_reduce_kernel(const double2* __restrict__ part, int64_t n, int S, double2* __restrict__ out) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        double2 a = part[e];
+        for (int y = 1; y < S; ++y) {
+            const double2 b = part[(int64_t)y * n + e];
+            a.x += b.x;
+            a.y += b.y;
+        }
+        out[e] = a;
+    }
 }
 
 // dense Kernel3d::kernel block: out(i,j) = G(trg_i, src_j), column-major M x N
@@ -894,10 +1047,10 @@ hipError_t launch_stage_t(const DevPlan& d, const Stage& s, int nrhs, hipStream_
         break;
     }
     case ST_GATHER:
-        gather_kernel<V><<<grid_for(d.N * nrhs, 256), 256, 0, st>>>(d.den_io, (V*)d.bt.p[BUF_DEN], d.perm, d.N, nrhs);
+        gather_kernel<V><<<grid_for(d.N * nrhs, 256), 256, 0, st>>>(d.io, (V*)d.bt.p[BUF_DEN], d.perm, d.N, nrhs);
         break;
     case ST_SCATTER:
-        scatter_kernel<V><<<grid_for(d.N * nrhs, 256), 256, 0, st>>>((const V*)d.bt.p[BUF_VAL], d.val_io, d.perm, d.N,
+        scatter_kernel<V><<<grid_for(d.N * nrhs, 256), 256, 0, st>>>((const V*)d.bt.p[BUF_VAL], d.io, d.perm, d.N,
                                                                      nrhs);
         break;
     case ST_KAPPLY: {
@@ -926,16 +1079,31 @@ hipError_t launch_stage_t(const DevPlan& d, const Stage& s, int nrhs, hipStream_
     case ST_FFTFWD:
         if (d.P != 6) return hipErrorInvalidValue;
         fft_fwd_kernel<V><<<dim3(s.stream ? std::min(n, AUX_BLOCKS) : n, nrhs), 256, 0, st>>>(d.fsrc, d.grid, d.NL,
-                                                                                              d.bt, s.begin, n);
+                                                                                              d.bt, s.begin, n, nrhs);
         break;
     case ST_VLIST: {
         if (d.P != 6) return hipErrorInvalidValue;
         if (n == 0) break;
-        constexpr int F = VSlice<V>::F;
         const int nchunk = (int)s.count;
-        vlist_acc_kernel<V><<<dim3((1728 / F) * nchunk, nrhs), 256, 0, st>>>(
-            d.vtrg, d.vchunk, d.vsrc, d.vpk, d.mixed ? (const V*)d.opmat32 : (const V*)d.opmat, d.bt, s.begin, s.buf,
-            nchunk);
+        const V* opm = d.mixed ? (const V*)d.opmat32 : (const V*)d.opmat;
+        if (nrhs == 1) {
+            constexpr int F = VSlice<V>::F;
+            vlist_acc_kernel<V><<<dim3((1728 / F) * nchunk, 1), 256, 0, st>>>(d.vtrg, d.vchunk, d.vsrc, d.vpk, opm,
+                                                                              d.bt, s.begin, s.buf, nchunk);
+        } else {
+            // right-hand sides per block: 16 / 8 / 4 (FP32) or 8 / 4 / 2 (FP64) by nrhs
+            constexpr int R2 = VMulti<V>::R2;
+            if (nrhs >= 8 * R2) {
+                vlist_acc_multi_kernel<V, 1, 8><<<dim3(1728 * nchunk, (nrhs + 8 * R2 - 1) / (8 * R2)), 256, 0, st>>>(
+                    d.vtrg, d.vchunk, d.vsrc, d.vpk, opm, d.bt, s.begin, s.buf, nchunk, nrhs);
+            } else if (nrhs >= 4 * R2) {
+                vlist_acc_multi_kernel<V, 2, 4><<<dim3(864 * nchunk, (nrhs + 4 * R2 - 1) / (4 * R2)), 256, 0, st>>>(
+                    d.vtrg, d.vchunk, d.vsrc, d.vpk, opm, d.bt, s.begin, s.buf, nchunk, nrhs);
+            } else {
+                vlist_acc_multi_kernel<V, 4, 2><<<dim3(432 * nchunk, (nrhs + 2 * R2 - 1) / (2 * R2)), 256, 0, st>>>(
+                    d.vtrg, d.vchunk, d.vsrc, d.vpk, opm, d.bt, s.begin, s.buf, nchunk, nrhs);
+            }
+        }
         vlist_inv_kernel<V><<<dim3(s.stream ? std::min(n, AUX_BLOCKS) : n, nrhs), 256, 0, st>>>(d.vtrg, d.grid, d.NL,
                                                                                                 d.bt, s.begin, n);
         break;
@@ -949,9 +1117,19 @@ hipError_t launch_stage_t(const DevPlan& d, const Stage& s, int nrhs, hipStream_
     return hipGetLastError();
 }
 
+__global__ void set_io_kernel(IoPtrs* io, const double2* den, double2* val) {
+    io->den = den;
+    io->val = val;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- launchers
+hipError_t launch_set_io(IoPtrs* io, const double2* den, double2* val, hipStream_t st) {
+    set_io_kernel<<<1, 1, 0, st>>>(io, den, val);
+    return hipGetLastError();
+}
+
 hipError_t launch_stage(const DevPlan& d, const Stage& s, int nrhs, hipStream_t st) {
     return d.mixed ? launch_stage_t<float2>(d, s, nrhs, st) : launch_stage_t<double2>(d, s, nrhs, st);
 }
@@ -975,9 +1153,37 @@ hipError_t launch_to_f32(const double2* in, float2* out, int64_t n, hipStream_t 
     return hipGetLastError();
 }
 
+DirectSplit direct_split(int M, int Ns, int nrhs) {
+    DirectSplit d;
+    d.rc = nrhs == 1 ? 1 : nrhs <= 4 ? 4 : 8;
+    const int mb = (M + 255) / 256, rg = (nrhs + d.rc - 1) / d.rc;
+    const int tiles = std::max(1, (Ns + 255) / 256);
+    // about 1024 blocks: source ranges until the target tiles and groups alone fill that;
+    // S * nrhs * M <= 1024 * 256 * 8 elements (32 MiB of partial sums)
+    int S = std::max(1, std::min(tiles, 1024 / std::max(1, mb * rg)));
+    const int per = (tiles + S - 1) / S;
+    d.chunk = per * 256;
+    d.S = (std::max(Ns, 1) + d.chunk - 1) / d.chunk;
+    d.part_elems = d.S > 1 ? (int64_t)d.S * nrhs * M : 0;
+    return d;
+}
+
 hipError_t launch_direct(const double* trg, int M, const double* src, int Ns, const double2* den, int nrhs,
-                         double2* out, int ktype, hipStream_t st) {
-    direct_kernel<<<dim3((M + 255) / 256, nrhs), 256, 0, st>>>(trg, M, src, Ns, den, nrhs, out, ktype);
+                         double2* out, double2* part, int ktype, hipStream_t st) {
+    const DirectSplit d = direct_split(M, Ns, nrhs);
+    if (d.S > 1 && !part) return hipErrorInvalidValue;
+    double2* dst = d.S > 1 ? part : out;
+    const dim3 grid((M + 255) / 256, d.S, (nrhs + d.rc - 1) / d.rc);
+    if (d.rc == 1)
+        direct_kernel<1><<<grid, 256, 0, st>>>(trg, M, src, Ns, den, nrhs, dst, d.chunk, ktype);
+    else if (d.rc == 4)
+        direct_kernel<4><<<grid, 256, 0, st>>>(trg, M, src, Ns, den, nrhs, dst, d.chunk, ktype);
+    else
+        direct_kernel<8><<<grid, 256, 0, st>>>(trg, M, src, Ns, den, nrhs, dst, d.chunk, ktype);
+    if (d.S > 1) {
+        const int64_t n = (int64_t)nrhs * M;
+        direct_reduce_kernel<<<grid_for(n, 256), 256, 0, st>>>(part, n, d.S, out);
+    }
     return hipGetLastError();
 }
 

@@ -150,7 +150,7 @@ struct Stage {
     double work;        // per RHS: GEMM useful flops, KAPPLY interactions, FFT spectra, V-list pairs,
                         // reduction elements read
     int mb;             // ST_GEMM: row blocks of every tile in the range (one kernel instance)
-    int pad2;
+    int lstage;         // ST_GEMM: logical stage (all tile heights of one planned GEMM stage), else -1
     double exec;        // ST_GEMM: MFMA flops per RHS the kernel executes (padding included)
     char name[24];
 };

@@ -283,7 +283,7 @@ struct Builder {
         if (kind != ST_ZERO && kind != ST_GATHER && kind != ST_SCATTER && e <= b) return;
         Stage s;
         std::memset(&s, 0, sizeof(s));
-        s.kind = kind; s.begin = b; s.end = e; s.buf = buf; s.count = count;
+        s.kind = kind; s.begin = b; s.end = e; s.buf = buf; s.count = count; s.lstage = -1;
         std::strncpy(s.name, name, sizeof(s.name) - 1);
         // solves: LF pinv factors and HF E1/E2/E3 chains
         const std::string nm(name);
@@ -291,8 +291,18 @@ struct Builder {
                   nm.find("hf_up_e") != std::string::npos || nm.find("hf_dn_e") != std::string::npos)
                      ? PREC_SOLVE : PREC_TRANSLATION;
         if (kind == ST_GEMM) {
+            // the logical stage keeps its groups and its tiling, so that the tiles can be
+            // rebuilt with another height limit (retile)
+            LogicalGemm lg;
+            for (int t = b; t < e; ++t)
+                if (lg.groups.empty() || lg.groups.back() != pl.tiles[t].g) lg.groups.push_back(pl.tiles[t].g);
+            std::strncpy(lg.name, name, sizeof(lg.name) - 1);
             resplit_rows(b, e);
             s.end = (int)pl.tiles.size();
+            lg.mbmax = 0;
+            for (int t = b; t < s.end; ++t) lg.mbmax = std::max(lg.mbmax, pl.tiles[t].mb);
+            s.lstage = (int)pl.lgemm.size();
+            pl.lgemm.push_back(lg);
             push_gemm_stages(pl.stages, s);
             return;
         }
@@ -927,7 +937,7 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
         if ((int)pl.tiles.size() > g0) {
             Stage s;
             std::memset(&s, 0, sizeof(s));
-            s.kind = ST_GEMM; s.begin = g0; s.end = (int)pl.tiles.size(); s.prec = PREC_SOLVE;
+            s.kind = ST_GEMM; s.begin = g0; s.end = (int)pl.tiles.size(); s.prec = PREC_SOLVE; s.lstage = -1;
             std::strncpy(s.name, "setup_fuse", sizeof(s.name) - 1);
             push_gemm_stages(pl.setup_stages, s);
         }
@@ -1432,53 +1442,7 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
     merge_sections();
 
     // ---------------- per-stage algorithmic work (per RHS)
-    for (auto& s : pl.stages) {
-        double w = 0;
-        if (s.kind == ST_GEMM) {
-            // useful flops of each tile: its rows x (K, or the present K blocks) x columns
-            for (int t = s.begin; t < s.end; ++t) {
-                const GTile& tl = pl.tiles[t];
-                const GGroup& g = pl.groups[tl.g];
-                const double rows = std::min(16 * tl.mb, g.M - tl.m0);
-                const int n1 = std::min(tl.n0 + GEMM_TN, g.ncols);
-                double k = 0;
-                if (g.kb == 0) {
-                    k = (double)g.K * (n1 - tl.n0);
-                } else {
-                    const int nkb = g.K / g.kb;
-                    for (int n = tl.n0; n < n1; ++n)
-                        for (int h = 0; h < nkb; ++h) k += pl.colB[g.bcol0 + (int64_t)n * nkb + h] >= 0 ? g.kb : 0;
-                }
-                w += 8.0 * rows * k;
-                // executed: per segment (K, or each K block present in some column), every
-                // wave with a column holding it runs ceil(len / 4) k steps of 4 * mb MFMAs
-                const int nkb = g.kb ? g.K / g.kb : 1;
-                const int len = g.kb ? g.kb : g.K;
-                for (int h = 0; h < nkb; ++h) {
-                    int waves = 0;
-                    for (int wv = 0; wv < 4; ++wv) {
-                        bool run = false;
-                        for (int n = tl.n0 + 16 * wv; n < std::min(tl.n0 + 16 * wv + 16, g.ncols) && !run; ++n)
-                            run = g.kb == 0 || pl.colB[g.bcol0 + (int64_t)n * nkb + h] >= 0;
-                        waves += run;
-                    }
-                    s.exec += waves * (double)((len + 3) / 4) * 4 * tl.mb * 2048.0;
-                }
-            }
-        } else if (s.kind == ST_KAPPLY) {
-            for (int t = s.begin; t < s.end; ++t) {
-                const KTask& k = pl.ktasks[t];
-                for (int q = k.s0; q < k.s1; ++q) w += (double)k.count * pl.ksrcs[q].count;
-            }
-        } else if (s.kind == ST_FFTFWD) {
-            w = s.end - s.begin;
-        } else if (s.kind == ST_VLIST) {
-            for (int t = s.begin; t < s.end; ++t) w += pl.vtrg[t].p1 - pl.vtrg[t].p0;
-        } else if (s.kind == ST_REDUCE) {
-            for (int t = s.begin; t < s.end; ++t) w += (double)pl.rtrg[t].cnt * pl.rtrg[t].M;
-        }
-        s.work = w;
-    }
+    for (auto& st : pl.stages) stage_work(pl, st);
 
     validate();
 

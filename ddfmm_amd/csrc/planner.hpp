@@ -23,6 +23,15 @@ struct PlanStats {
     int nlevels_lf = 0, nlevels_hf = 0, max_hf_width = 0;
 };
 
+// A planned GEMM stage before it is cut into one launch per tile height: its groups, in
+// order, and the largest tile height its tiling uses (GEMM_MB, or less where rows were
+// split for a small stage).
+struct LogicalGemm {
+    std::vector<int> groups;
+    int mbmax = GEMM_MB;
+    char name[24] = {0};
+};
+
 struct HostPlan {
     Params prm;
     int64_t N = 0;
@@ -49,6 +58,7 @@ struct HostPlan {
     std::vector<RTarget> rtrg;
     std::vector<Stage> stages;
     std::vector<Stage> setup_stages;       // FP64 GEMMs on the operator table, run once after generation
+    std::vector<LogicalGemm> lgemm;        // logical GEMM stages of `stages` (Stage::lstage)
     int64_t bufsize[BUF_COUNT] = {0};      // elements per rhs
     PlanStats stats;
     std::string dump;                      // canonical text of the lists (debug/parity)
@@ -56,5 +66,15 @@ struct HostPlan {
 
 void build_plan(const Params& prm, const double* pos, int64_t N, const Mlib& mlib,
                 HostPlan& plan, bool want_dump);
+
+// Algorithmic work per RHS of a stage (Stage::work) and, for GEMM stages, the MFMA flops
+// the kernel executes under its tiling (Stage::exec).
+void stage_work(const HostPlan& plan, Stage& s);
+
+// Rebuild the tiles of every logical GEMM stage with at most mbmax[l] row blocks of 16 per
+// tile (1..GEMM_MB; a group's rows spread evenly over ceil(blocks / mbmax) tiles) and the
+// GEMM launches of `stages` (one per tile height), after setup: the setup-time GEMMs'
+// tiles are dropped.  Other stages keep their order.
+void retile(HostPlan& plan, const std::vector<int>& mbmax);
 
 }  // namespace ddfmm

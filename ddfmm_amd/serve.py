@@ -62,11 +62,15 @@ class Request:
         return None if self.t_done is None else self.t_done - self.arrival
 
 
+_TILE_CACHE: dict = {}
+
+
 class GpuEngine:
     """One geometry resident on the device: a Wave3d context, a HIP stream, double-buffered
     pinned host and device staging (nrhs_slot x N complex each)."""
 
-    def __init__(self, key, pos, wave_opts, ops, nrhs_slot, precision, kernel=0, numC=1, hip=None):
+    def __init__(self, key, pos, wave_opts, ops, nrhs_slot, precision, kernel=0, numC=1, hip=None,
+                 autotune=True, graph=True):
         from ._lib import Hip
         from .wave3d import Kernel3d, Mlib3d, Wave3d
 
@@ -80,6 +84,19 @@ class GpuEngine:
         self.w.posptr, self.w.mlibptr, self.w.kernel = pos, ml, Kernel3d(kernel)
         self.w.geomprtn = np.zeros((numC,) * 3, dtype=np.int32)
         self.w.setup(wave_opts, nrhs_max=nrhs_slot, precision=precision)
+        # GEMM tile heights: tuned once per (geometry, slot, precision, kernel) in this
+        # process and reapplied when the geometry is admitted again after an eviction
+        self.tiles = None
+        if autotune:
+            ck = (key, nrhs_slot, precision, kernel)
+            self.tiles = _TILE_CACHE.get(ck)
+            if self.tiles is None:
+                self.tiles = self.w.autotune(nrhs_slot, reps=1)["tiles"]
+                _TILE_CACHE[ck] = self.tiles
+            else:
+                self.w.set_tiles(self.tiles)
+        if graph:
+            self.w.set_graph(True)
         self.stream = self.hip.stream()
         nb = nrhs_slot * self.N * 16
         self.slots = []

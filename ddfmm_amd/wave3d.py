@@ -38,7 +38,11 @@ class Kernel3d:
 
 
 class Mlib3d:
-    """Operator-library handle (mlib3d.hpp:58-99): option keys -<prefix>NPQ/ldname/hdname."""
+    """Operator-library handle (mlib3d.hpp:58-99): option keys -<prefix>NPQ/ldname/hdname.
+
+    setup() reads the two files once into a library handle (ddfmm_mlib_load) that every
+    Wave3d set up with this object shares; ``w2ldmap`` / ``w2hdmap`` list the widths of
+    the LF and HF tables."""
 
     def __init__(self, prefix: str = "mlib3d_"):
         self.prefix = prefix
@@ -46,6 +50,7 @@ class Mlib3d:
         self.NPQ = 4
         self.ldname = ""
         self.hdname = ""
+        self._h = None
 
     def setup(self, opts: dict) -> int:
         p = "-" + self.prefix
@@ -53,10 +58,43 @@ class Mlib3d:
             self.NPQ = int(opts[p + "NPQ"])
         self.ldname = str(opts.get(p + "ldname", self.ldname))
         self.hdname = str(opts.get(p + "hdname", self.hdname))
-        for f in (self.ldname, self.hdname):
-            with open(f, "rb"):
-                pass
+        self.close()
+        h = ctypes.c_void_p()
+        check(lib().ddfmm_mlib_load(self.ldname.encode(), self.hdname.encode(), ctypes.byref(h)))
+        self._h = h
         return 0
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise RuntimeError("Mlib3d.setup has not been called")
+        return self._h
+
+    def _widths(self):
+        nld, nhd = ctypes.c_int(), ctypes.c_int()
+        ld = (ctypes.c_double * 64)()
+        hd = (ctypes.c_double * 64)()
+        check(lib().ddfmm_mlib_widths(self.handle, 64, ld, ctypes.byref(nld), hd, ctypes.byref(nhd)))
+        return list(ld[:nld.value]), list(hd[:nhd.value])
+
+    @property
+    def w2ldmap(self):
+        return {w: None for w in self._widths()[0]}
+
+    @property
+    def w2hdmap(self):
+        return {w: None for w in self._widths()[1]}
+
+    def close(self):
+        if self._h is not None:
+            lib().ddfmm_mlib_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _opt(opts, key, cast, default):
@@ -119,8 +157,7 @@ class Wave3d:
         check(lib().ddfmm_create(self.device, ctypes.byref(h)))
         self._ctx = h
         self._n = pos.shape[0]
-        check(lib().ddfmm_setup(h, ptr(pos), pos.shape[0], ctypes.byref(prm),
-                                self.mlibptr.ldname.encode(), self.mlibptr.hdname.encode()))
+        check(lib().ddfmm_setup_mlib(h, ptr(pos), pos.shape[0], ctypes.byref(prm), self.mlibptr.handle))
         return 0
 
     def eval(self, den: np.ndarray) -> np.ndarray:
@@ -162,6 +199,37 @@ class Wave3d:
 
     def set_timing(self, on: bool = True):
         check(lib().ddfmm_set_timing(self._ctx, int(on)))
+
+    def autotune(self, nrhs: int = 1, reps: int = 2) -> dict:
+        """Pick the tile-height limit of every GEMM stage by timing (ddfmm_autotune)."""
+        cnt = ctypes.c_int()
+        check(lib().ddfmm_get_tiles(self._ctx, 0, None, None, ctypes.byref(cnt)))
+        n = cnt.value
+        table = (ctypes.c_int * max(n, 1))()
+        before, after = ctypes.c_double(), ctypes.c_double()
+        check(lib().ddfmm_autotune(self._ctx, int(nrhs), int(reps), n, table, ctypes.byref(cnt),
+                                   ctypes.byref(before), ctypes.byref(after)))
+        return {"tiles": list(table[:n]), "gemm_ms_planner": before.value, "gemm_ms_tuned": after.value}
+
+    def tiles(self) -> tuple[list, list]:
+        """(per-stage tile-height limits, stage names) of the current GEMM tiling."""
+        cnt = ctypes.c_int()
+        check(lib().ddfmm_get_tiles(self._ctx, 0, None, None, ctypes.byref(cnt)))
+        n = cnt.value
+        table = (ctypes.c_int * max(n, 1))()
+        names = ctypes.create_string_buffer(24 * max(n, 1))
+        check(lib().ddfmm_get_tiles(self._ctx, n, table, names, ctypes.byref(cnt)))
+        raw = names.raw
+        return list(table[:n]), [raw[24 * i:24 * i + 24].split(b"\0")[0].decode() for i in range(n)]
+
+    def set_tiles(self, table) -> None:
+        t = (ctypes.c_int * max(len(table), 1))(*table)
+        check(lib().ddfmm_set_tiles(self._ctx, len(table), t))
+
+    def set_graph(self, on: bool = True):
+        """Capture the evaluation of each right-hand-side count once into a HIP graph and
+        replay it (ddfmm_set_graph)."""
+        check(lib().ddfmm_set_graph(self._ctx, int(on)))
 
     def stage_times(self):
         """[(name, ms, work_per_rhs, kind, precision_class)] of the last timed eval."""

@@ -164,3 +164,51 @@ def test_blocked_gemm_padding_stays_small(oplib):
     l2l_w1 = sum(s[1] for s in st[last_red:] if s[0] == "hf_l2l")
     lf_red = sum(s[1] for s in st if s[0] == "lf_reduce")
     assert lf_red > 0 and l2l_w1 / (8.0 * 128) / lf_red >= 3.0
+
+
+def test_retile_keeps_useful_work(oplib):
+    """Rebuilding the GEMM tiles with another tile-height limit (the autotuner's search
+    space) changes the launches and the executed MFMA work, never the useful work."""
+    from ddfmm_amd._lib import Stats
+
+    pos = geometry.sphere(6000, 4.0, 0)
+    prm = _lib.Params(K=4.0, ACCU=2, NPQ=4, ptsmax=100, maxlevel=12, numC=1, kernel=0, nrhs_max=1, precision=1)
+    h = ctypes.c_void_p()
+    L = _lib.lib()
+    _lib.check(L.ddfmm_create(-1, ctypes.byref(h)))
+    try:
+        _lib.check(L.ddfmm_setup(h, pos.ctypes.data, len(pos), ctypes.byref(prm), oplib[0].encode(),
+                                 oplib[1].encode()))
+
+        def work():
+            n = ctypes.c_int()
+            _lib.check(L.ddfmm_stage_times(h, 0, None, None, None, None, ctypes.byref(n)))
+            w = (ctypes.c_double * n.value)()
+            k = (ctypes.c_int * n.value)()
+            ex = (ctypes.c_double * n.value)()
+            _lib.check(L.ddfmm_stage_times(h, n.value, None, None, w, k, ctypes.byref(n)))
+            _lib.check(L.ddfmm_stage_exec(h, n.value, ex, ctypes.byref(n)))
+            gemm = [i for i in range(n.value) if k[i] % 16 == 4]
+            return sum(w[i] for i in gemm), sum(ex[i] for i in gemm), len(gemm)
+
+        cnt = ctypes.c_int()
+        _lib.check(L.ddfmm_get_tiles(h, 0, None, None, ctypes.byref(cnt)))
+        nl = cnt.value
+        assert nl >= 3
+        u0, e0, n0 = work()
+        for m in (1, 3, 7):
+            tab = (ctypes.c_int * nl)(*([m] * nl))
+            _lib.check(L.ddfmm_set_tiles(h, nl, tab))
+            got = (ctypes.c_int * nl)()
+            _lib.check(L.ddfmm_get_tiles(h, nl, got, None, ctypes.byref(cnt)))
+            assert list(got) == [m] * nl
+            u, e, n = work()
+            assert u == pytest.approx(u0, rel=1e-12)
+            assert e >= u
+        bad = (ctypes.c_int * nl)(*([8] * nl))
+        with pytest.raises(_lib.DdfmmError, match="out of range"):
+            _lib.check(L.ddfmm_set_tiles(h, nl, bad))
+        with pytest.raises(_lib.DdfmmError, match="entries"):
+            _lib.check(L.ddfmm_set_tiles(h, nl - 1, bad))
+    finally:
+        L.ddfmm_destroy(h)

@@ -1,0 +1,172 @@
+"""Every benchmark configuration at its real size on the GPU, checked at 1000 seeded
+targets against the FP64 direct sum (relative L2 error <= 1e-5 for e^{ikr}/r), plus an
+adaptive geometry whose leaves reach the 152-point threshold of the W and X lists
+(wave3d_eval.cpp:663-718), checked against the oracle.
+
+Right-hand sides are computed independently (no atomics, fixed summation order), so a
+right-hand side evaluated alone or inside a batch gives the same bits.
+"""
+import numpy as np
+import pytest
+
+from ddfmm_amd import geometry
+from ddfmm_amd.wave3d import FP64, FP64_FUSED, MIXED, Kernel3d, Mlib3d, Wave3d, direct_sum
+from oracle import wave3d_ref as R
+
+pytestmark = pytest.mark.gpu
+
+FMM_TOL = 1e-5
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _wave(workload, oplib, precision, nrhs_max=None, kernel=0):
+    wl = geometry.WORKLOADS[workload]
+    pos = geometry.workload_points(workload)
+    ml = Mlib3d()
+    ml.setup({"-mlib3d_ldname": oplib[0], "-mlib3d_hdname": oplib[1]})
+    w = Wave3d()
+    w.posptr, w.mlibptr, w.kernel = pos, ml, Kernel3d(kernel)
+    w.geomprtn = np.zeros((wl.get("numC", 1),) * 3, dtype=np.int32)
+    w.setup({"-wave3d_ACCU": 2, "-wave3d_K": wl["K"], "-wave3d_ptsmax": wl["ptsmax"], "-wave3d_maxlevel": 14},
+            nrhs_max=nrhs_max or wl["nrhs"], precision=precision)
+    return w, pos, wl
+
+
+def _check(pos, val, den, seed, kernel=0, n=1000):
+    chk = geometry.check_targets(len(pos), n, seed)
+    exact = direct_sum(pos[chk], pos, np.atleast_2d(den), kernel)
+    return rel(np.atleast_2d(val)[:, chk], exact)
+
+
+def test_config1_sphere32_1m(oplib):
+    """1M points on the d = 32 sphere, 1 RHS: mixed and plain FP64 within 1e-5 of the
+    direct sum, and within 3e-6 of each other."""
+    w, pos, wl = _wave("sphere32_1m", oplib, MIXED)
+    den = geometry.densities(len(pos), wl["seed"])
+    vm = w.eval(den)
+    w.close()
+    w, _, _ = _wave("sphere32_1m", oplib, FP64)
+    vf = w.eval(den)
+    w.close()
+    em, ef = _check(pos, vm, den, wl["seed"]), _check(pos, vf, den, wl["seed"])
+    d = rel(vm, vf)
+    print(f"config1: mixed {em:.3e} fp64 {ef:.3e} mixed-vs-fp64 {d:.3e}", flush=True)
+    assert em < FMM_TOL and ef < FMM_TOL and d < 3e-6
+
+
+def test_config2_sphere16_250k_x64(oplib):
+    """250k points, 64 RHS (seeds of the workload), e^{ikr}/r, mixed: every RHS within 1e-5
+    of the direct sum at 1000 targets; RHS alone and in subsets give the batch's bits."""
+    w, pos, wl = _wave("sphere16_250k_x64", oplib, MIXED)
+    den = geometry.densities(len(pos), wl["seed"], nrhs=64)
+    val = w.eval(den)
+    e = _check(pos, val, den, wl["seed"])
+    print(f"config2 helm mixed x64: {e:.3e}", flush=True)
+    assert e < FMM_TOL
+    for r in (0, 17, 63):
+        assert np.array_equal(w.eval(den[r]), val[r]), r
+    assert np.array_equal(w.eval(den[5:13]), val[5:13])      # 8 RHS: the 8-RHS block shapes
+    assert np.array_equal(w.eval(den[40:43]), val[40:43])    # 3 RHS: the 4-RHS block shapes
+    w.close()
+
+
+def test_config2_expr_mixed_vs_fp64(oplib_expr):
+    """e^{ikr} on the same geometry: the ACCU = 2 method limit for this kernel is about
+    5e-2 against the direct sum (the LF far field of a kernel that solves no PDE); the
+    mixed-precision path stays within 1e-4 of the FP64 path."""
+    nr = 16
+    w, pos, wl = _wave("sphere16_250k_x64", oplib_expr, MIXED, nrhs_max=nr, kernel=1)
+    den = geometry.densities(len(pos), wl["seed"], nrhs=nr)
+    vm = w.eval(den)
+    w.close()
+    w, _, _ = _wave("sphere16_250k_x64", oplib_expr, FP64, nrhs_max=nr, kernel=1)
+    vf = w.eval(den)
+    w.close()
+    d = rel(vm, vf)
+    e = _check(pos, vf, den, wl["seed"], kernel=1)
+    print(f"config2 expr x{nr}: mixed-vs-fp64 {d:.3e}, fp64 vs direct {e:.3e}", flush=True)
+    assert d < 1e-4
+    assert e < 8e-2
+
+
+def test_config3_ellipsoid_4m_x8(oplib):
+    """4M points on the perturbed ellipsoid, 8 RHS, ptsmax = 400: the adaptive tree uses
+    the W list through equivalent densities and the X list into check values."""
+    w, pos, wl = _wave("ellipsoid_4m_x8", oplib, MIXED)
+    st = w.stats()
+    assert st["x2chk_interactions"] > 0
+    den = geometry.densities(len(pos), wl["seed"], nrhs=8)
+    val = w.eval(den)
+    e = _check(pos, val, den, wl["seed"])
+    print(f"config3 mixed x8: {e:.3e}; leaves {st['nleaf']}, x2chk {st['x2chk_interactions']}", flush=True)
+    assert e < FMM_TOL
+    assert np.array_equal(w.eval(den[3]), val[3])
+    w.close()
+
+
+def _adaptive_points():
+    # sphere of diameter 3.4 in K = 4 with 20k area-uniform points plus a 4x denser cap
+    # (z > 1.2): with ptsmax = 400 the W = 1/2 leaves of the sparse part hold up to 400
+    # points next to the split boxes of the cap, whose W = 1/4 boxes reach 152 points, so
+    # both >= 152-point branches (W through equivalents, X into check values) have work
+    a = geometry.sphere(20000, 3.4, 1)
+    b = geometry.sphere(80000, 3.4, 2)
+    return np.concatenate([a, b[b[:, 2] > 1.2]]), 4.0
+
+
+def test_adaptive_w_x_branches_vs_oracle(oplib, ops):
+    """The >= 152-point branches of the W list (source equivalents, wave3d_eval.cpp:705-716)
+    and the X list (sources into check values, 675-683) run on the GPU and match the
+    oracle; the FP64 path to 1e-9, the mixed path to the direct sum within 1e-5."""
+    pos, K = _adaptive_points()
+    den = geometry.densities(len(pos), 9, nrhs=2)
+    print(f"adaptive geometry: {len(pos)} points", flush=True)
+    ora = R.Wave3dOracle(pos, K, *ops, ptsmax=400).setup()
+    live = {k: b for k, b in ora.boxes.items() if b.tag & R.PTS}
+    NL = 152
+    w_equiv = sum(1 for b in live.values() for s in b.W
+                  if not (live[s].tag & R.TERMINAL and len(live[s].ptidx) < NL))
+    x_chk = sum(1 for k, b in live.items() for s in b.X
+                if not (b.tag & R.TERMINAL and len(b.ptidx) < NL))
+    assert w_equiv > 0 and x_chk > 0, (w_equiv, x_chk)
+    ref = np.stack([ora.eval(den[r]) for r in range(2)])
+    for prec, tol in ((FP64, 1e-9), (FP64_FUSED, 1e-7), (MIXED, 3e-6)):
+        ml = Mlib3d()
+        ml.setup({"-mlib3d_ldname": oplib[0], "-mlib3d_hdname": oplib[1]})
+        w = Wave3d()
+        w.posptr, w.mlibptr = pos, ml
+        w.setup({"-wave3d_ACCU": 2, "-wave3d_K": K, "-wave3d_ptsmax": 400, "-wave3d_maxlevel": 12},
+                nrhs_max=2, precision=prec)
+        st = w.stats()
+        assert st["x2chk_interactions"] > 0
+        val = w.eval(den)
+        e = rel(val, ref)
+        ed = _check(pos, val, den, 9, n=500)
+        print(f"adaptive prec={prec}: vs oracle {e:.3e}, vs direct {ed:.3e} (W equiv {w_equiv}, X chk {x_chk})",
+              flush=True)
+        assert e < tol and ed < FMM_TOL
+        w.close()
+
+
+@pytest.mark.parametrize("N,K,seed", [(6000, 4.0, 0), (60000, 8.0, 0)])
+def test_fp64_fused_vs_oracle_and_direct(oplib, ops, N, K, seed):
+    """FP64 with the folded solve matrices (the mixed mode's plan in FP64): within 1e-7 of
+    the oracle and 1e-5 of the direct sum; a batch of 3 RHS gives each RHS's bits."""
+    pos = geometry.sphere(N, K, seed)
+    den = geometry.densities(N, seed, nrhs=3)
+    ml = Mlib3d()
+    ml.setup({"-mlib3d_ldname": oplib[0], "-mlib3d_hdname": oplib[1]})
+    w = Wave3d()
+    w.posptr, w.mlibptr = pos, ml
+    w.setup({"-wave3d_ACCU": 2, "-wave3d_K": K, "-wave3d_maxlevel": 12}, nrhs_max=3, precision=FP64_FUSED)
+    val = w.eval(den)
+    ref = R.Wave3dOracle(pos, K, *ops).setup().eval(den[1])
+    e = rel(val[1], ref)
+    ed = _check(pos, val, den, seed, n=500)
+    print(f"fp64_fused N={N}: vs oracle {e:.3e}, vs direct {ed:.3e}", flush=True)
+    assert e < 1e-7 and ed < FMM_TOL
+    assert np.array_equal(w.eval(den[2]), val[2])
+    w.close()

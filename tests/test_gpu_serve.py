@@ -79,3 +79,42 @@ def test_tt_driver_reference_formats(tmp_path):
     chk = geometry.check_targets(len(pos), 200, 0)
     exact = R.direct_sum(pos[chk], pos, den)
     assert np.linalg.norm(val[chk] - exact) / np.linalg.norm(exact) < 1e-5
+
+
+def test_config4_serving_mix_real_geometries(oplib):
+    """The serving mix over the four benchmark geometries (60k-4M points) resident at once:
+    requests of 1-4 RHS batched per geometry; one request per geometry replayed alone gives
+    the same bits as in the mix, and each result is within 1e-5 of the direct sum."""
+    from ddfmm_amd.wave3d import direct_sum
+
+    names = ["sphere8_60k", "sphere16_250k_x64", "sphere32_1m", "ellipsoid_4m_x8"]
+    pts = {n: geometry.workload_points(n) for n in names}
+
+    def make(key):
+        wl = geometry.WORKLOADS[key]
+        return GpuEngine(key, pts[key], {"-wave3d_ACCU": 2, "-wave3d_K": wl["K"], "-wave3d_ptsmax": wl["ptsmax"],
+                                         "-wave3d_maxlevel": 14}, oplib, 4, MIXED, numC=wl.get("numC", 1))
+
+    srv = Server(make, lambda key: 0, hbm_budget=200 << 30)
+    reqs = poisson_requests({n: len(pts[n]) for n in names}, qps=50, count=16, seed=21, max_rhs=4)
+    srv.run(reqs)
+    st = latency_stats(reqs)
+    print(f"mix: served {st['served']} in {srv.batches} batches, p50 {st['p50_ms']:.1f} ms "
+          f"p99 {st['p99_ms']:.1f} ms", flush=True)
+    assert all(r.error is None for r in reqs)
+    seen = set()
+    for r in reqs:
+        if r.geometry in seen:
+            continue
+        seen.add(r.geometry)
+        alone = Request(2000 + r.rid, r.geometry, r.den, 0.0)
+        srv.run([alone])
+        assert np.array_equal(alone.result, r.result), (r.geometry, r.rid)
+        P = pts[r.geometry]
+        chk = geometry.check_targets(len(P), 300, 21)
+        exact = direct_sum(P[chk], P, np.atleast_2d(r.den))
+        e = float(np.linalg.norm(np.atleast_2d(r.result)[:, chk] - exact) / np.linalg.norm(exact))
+        print(f"  {r.geometry} rid {r.rid} nrhs {r.nrhs}: alone == in mix, rel.err {e:.2e}", flush=True)
+        assert e < 1e-5
+    assert len(seen) == 4
+    srv.close()

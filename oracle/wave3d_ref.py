@@ -89,6 +89,17 @@ def relerr(approx: np.ndarray, exact: np.ndarray) -> float:
     return float(np.linalg.norm(approx - exact) / np.linalg.norm(exact))
 
 
+def _translate_fp64(A: np.ndarray, x: np.ndarray) -> np.ndarray:
+    return A @ x
+
+
+# Every translation matvec of eval (LF M2M/L2L, HF U2U/M2L/D2D; the reference's zgemv
+# calls in wave3d_eval.cpp:461, 578, 812, 831, 925, 967, 989) goes through
+# this hook, so that tests can substitute an emulated reduced-precision product; the
+# check-to-equivalent solves, the kernel applications and the FFT stay FP64.
+translate = _translate_fp64
+
+
 # ---------------------------------------------------------------- direction algebra
 def nml2dir(n, W: float, NPQ: int):
     """wave3d.cpp:43-71."""
@@ -440,7 +451,7 @@ class Wave3dOracle:
                     for ind in range(8):
                         ck = child_key(k, ind)
                         if ck in boxes:
-                            chk = chk + ue2uc[ind] @ upeq[ck]
+                            chk = chk + translate(ue2uc[ind], upeq[ck])
                 upeq[k] = V @ ((UH @ chk) * invs[:, 0])
         # HF pass (wave3d_eval.cpp:188-235)
         dirup, dirdn = {}, {}
@@ -519,7 +530,7 @@ class Wave3dOracle:
                     for ind in range(8):
                         ck = child_key(k, ind)
                         if ck in boxes:
-                            dnchk[ck] = dnchk.get(ck, 0) + de2dc[ind] @ dneq
+                            dnchk[ck] = dnchk.get(ck, 0) + translate(de2dc[ind], dneq)
         val = np.zeros(len(self.pos), dtype=np.complex128)
         for k, ev in extval.items():
             val[boxes[k].ptidx] = ev
@@ -538,7 +549,7 @@ class Wave3dOracle:
                 ck = child_key(k, ind)
                 if ck in boxes:
                     x = upeq[ck] if W == 1 else dirup[(ck, pd)]
-                    chk = chk + ue2uc[ind] @ x
+                    chk = chk + translate(ue2uc[ind], x)
             dirup[(k, d)] = E1 @ (E2 @ (E3 @ chk))
 
     def _down_high(self, W, d, lists, boxes, dirup, dirdn, dnchk):
@@ -560,7 +571,7 @@ class Wave3dOracle:
                 off = tuple(np.rint((c - self.center(sk)) / W).astype(np.int64))
                 if off not in mcache:
                     mcache[off] = kernel(dcp + np.asarray(off, dtype=np.float64) * W, uep, kt)
-                dcv = dcv + mcache[off] @ dirup.get((sk, d), np.zeros(len(uep), dtype=np.complex128))
+                dcv = dcv + translate(mcache[off], dirup.get((sk, d), np.zeros(len(uep), dtype=np.complex128)))
             dirdn[(k, d)] = dcv
             dneq = E3.T @ (E2.T @ (E1.T @ dcv))
             for ind in range(8):
@@ -568,9 +579,9 @@ class Wave3dOracle:
                 if ck not in boxes:
                     continue
                 if W == 1:
-                    dnchk[ck] = dnchk.get(ck, 0) + de2dc[ind] @ dneq
+                    dnchk[ck] = dnchk.get(ck, 0) + translate(de2dc[ind], dneq)
                 else:
-                    dirdn[(ck, pd)] = dirdn.get((ck, pd), 0) + de2dc[ind] @ dneq
+                    dirdn[(ck, pd)] = dirdn.get((ck, pd), 0) + translate(de2dc[ind], dneq)
 
 
 def dump_lists(w: "Wave3dOracle") -> str:

@@ -13,7 +13,7 @@ EXPORTS = ["ddfmm_last_error", "ddfmm_create", "ddfmm_destroy", "ddfmm_setup", "
            "ddfmm_eval_device", "ddfmm_direct_sum", "ddfmm_kernel_matrix", "ddfmm_get_stats",
            "ddfmm_dump_lists", "ddfmm_read_buffer", "ddfmm_set_timing", "ddfmm_stage_times",
            "ddfmm_stage_exec", "ddfmm_set_graph", "ddfmm_mlib_load", "ddfmm_mlib_free", "ddfmm_mlib_widths",
-           "ddfmm_setup_mlib", "ddfmm_autotune", "ddfmm_set_tiles", "ddfmm_get_tiles"]
+           "ddfmm_setup_mlib", "ddfmm_autotune", "ddfmm_set_tiles", "ddfmm_get_tiles", "ddfmm_direct_sum_noalloc"]
 
 
 class Params(ctypes.Structure):
@@ -64,6 +64,7 @@ def lib():
     L.ddfmm_setup_mlib.argtypes = [vp, vp, i64, ctypes.POINTER(Params), vp]
     L.ddfmm_eval_device.argtypes = [vp, vp, vp, ci, vp]
     L.ddfmm_direct_sum.argtypes = [ci, vp, ci, vp, ci, vp, ci, vp]
+    L.ddfmm_direct_sum_noalloc.argtypes = [ci, vp, ci, vp, ci, vp, ci, vp, vp]
     L.ddfmm_kernel_matrix.argtypes = [ci, vp, ci, vp, ci, vp]
     L.ddfmm_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
     L.ddfmm_dump_lists.argtypes = [vp, ctypes.c_char_p, i64, ctypes.POINTER(i64)]

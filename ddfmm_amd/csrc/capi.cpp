@@ -585,6 +585,28 @@ int ddfmm_direct_sum(int kernel, const double* trg, int m, const double* src, in
     return rc;
 }
 
+int ddfmm_direct_sum_noalloc(int kernel, const double* trg_dev, int m, const double* src_dev, int n,
+                             const double* den_dev, int nrhs, double* out_dev, void* stream) {
+    if (!trg_dev || !src_dev || !den_dev || !out_dev || m < 0 || n < 0 || nrhs < 1)
+        return fail(DDFMM_EINVAL, "bad argument");
+    if (kernel != 0 && kernel != 1) return fail(DDFMM_EINVAL, "Unknown kernel type " + std::to_string(kernel));
+    if (m == 0) return DDFMM_OK;
+    const DirectSplit sp = direct_split(m, n, nrhs);
+    if (sp.part_elems * (int64_t)sizeof(double2) > DIRECT_SCRATCH_BYTES)
+        return fail(DDFMM_EINVAL, "direct sum needs " + std::to_string(sp.part_elems * 16) +
+                                      " bytes of partial sums, more than the static " +
+                                      std::to_string(DIRECT_SCRATCH_BYTES) + "-byte scratch");
+    try {
+        double2* part = nullptr;
+        if (sp.part_elems > 0) HIPCHK(direct_static_scratch(&part));
+        HIPCHK(launch_direct(trg_dev, m, src_dev, n, (const double2*)den_dev, nrhs, (double2*)out_dev, part, kernel,
+                             (hipStream_t)stream));
+    } catch (const std::exception& e) {
+        return fail(DDFMM_EDEVICE, e.what());
+    }
+    return DDFMM_OK;
+}
+
 int ddfmm_kernel_matrix(int kernel, const double* trg, int m, const double* src, int n, double* out) {
     if (!trg || !src || !out || m < 0 || n < 0) return fail(DDFMM_EINVAL, "bad argument");
     if (kernel != 0 && kernel != 1) return fail(DDFMM_EINVAL, "Unknown kernel type " + std::to_string(kernel));

@@ -59,10 +59,12 @@ struct DirectSplit {
     int S = 1, chunk = 0, rc = 1;
     int64_t part_elems = 0;
 };
+constexpr int64_t DIRECT_SCRATCH_BYTES = int64_t(32) << 20;   // bound of part_elems * 16 bytes
 DirectSplit direct_split(int M, int Ns, int nrhs);
 hipError_t launch_direct(const double* trg, int M, const double* src, int Ns, const double2* den, int nrhs,
                          double2* out, double2* part, int ktype, hipStream_t st);
-
+// The static partial-sum array of the no-allocation direct sum on the current device.
+hipError_t direct_static_scratch(double2** out);
 hipError_t launch_kmatrix(const double* trg, int M, const double* src, int Ns, void* out, int ktype, int f32,
                           hipStream_t st);
 hipError_t launch_to_f32(const double2* in, float2* out, int64_t n, hipStream_t st);

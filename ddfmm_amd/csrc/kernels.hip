@@ -21,6 +21,9 @@
 // exp(ikr) = exp(2*pi*i*(r - rint(r)))).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <vector>
+
 #include "device.hpp"
 
 namespace ddfmm {
@@ -1159,7 +1162,7 @@ DirectSplit direct_split(int M, int Ns, int nrhs) {
     const int mb = (M + 255) / 256, rg = (nrhs + d.rc - 1) / d.rc;
     const int tiles = std::max(1, (Ns + 255) / 256);
     // about 1024 blocks: source ranges until the target tiles and groups alone fill that;
-    // S * nrhs * M <= 1024 * 256 * 8 elements (32 MiB of partial sums)
+    // S * nrhs * M <= 1024 * 256 * 8 elements, so the partial sums fit in DIRECT_SCRATCH_BYTES
     int S = std::max(1, std::min(tiles, 1024 / std::max(1, mb * rg)));
     const int per = (tiles + S - 1) / S;
     d.chunk = per * 256;
@@ -1194,6 +1197,30 @@ hipError_t launch_kmatrix(const double* trg, int M, const double* src, int Ns, v
     else
         kmatrix_kernel<double2><<<grid_for((int64_t)M * Ns, 256), 256, 0, st>>>(trg, M, src, Ns, (double2*)out, ktype);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- no-allocation direct sum
+// Partial sums of the direct sum for callers that cannot allocate: one static array per
+// GPU, its device address taken once per device with that device current.  Calls on one
+// GPU share the array, so calls that overlap on one GPU race.
+static __device__ __align__(256) double2 scratch[DIRECT_SCRATCH_BYTES / sizeof(double2)];
+
+hipError_t direct_static_scratch(double2** out) {
+    static std::mutex mu;
+    static std::vector<double2*> ptr;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)ptr.size() <= dev) ptr.resize(dev + 1, nullptr);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        ptr[dev] = (double2*)p;
+    }
+    *out = ptr[dev];
+    return hipSuccess;
 }
 
 }  // namespace ddfmm

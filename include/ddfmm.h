@@ -127,6 +127,17 @@ DDFMM_API int ddfmm_get_tiles(ddfmm_ctx* ctx, int cap, int* mbmax, char* names /
 DDFMM_API int ddfmm_direct_sum(int kernel, const double* trg, int m, const double* src, int n,
                      const double* den, int nrhs, double* out);
 
+/* The same direct sum on device buffers (trg m x 3, src n x 3, den nrhs x n complex,
+   out nrhs x m complex), queued on `stream` (0 = the null stream), with no device
+   allocation: the partial sums of its source ranges live in a static 32 MiB array of
+   the library on each GPU.  No counterpart in the reference: for callers that cannot
+   call hipMalloc (graph capture, allocation-free serving loops).  Results are
+   bit-identical to ddfmm_direct_sum.  Every call on one GPU shares that GPU's array,
+   so calls that overlap on one GPU race.  Sizes whose partial sums would exceed the
+   array are rejected with DDFMM_EINVAL (the source split keeps them within it). */
+DDFMM_API int ddfmm_direct_sum_noalloc(int kernel, const double* trg_dev, int m, const double* src_dev, int n,
+                             const double* den_dev, int nrhs, double* out_dev, void* stream);
+
 /* Dense kernel block (Kernel3d::kernel, kernel3d.cpp:26-107): out(i,j) = G(trg_i, src_j),
    m x n column-major complex; points as rows. */
 DDFMM_API int ddfmm_kernel_matrix(int kernel, const double* trg, int m, const double* src, int n, double* out);

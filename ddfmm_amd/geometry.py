@@ -83,6 +83,9 @@ WORKLOADS = {
     "sphere32_1m": dict(kind="sphere", n=1_000_000, diameter=32.0, K=32.0, seed=1, nrhs=1, ptsmax=100),
     "sphere16_250k_x64": dict(kind="sphere", n=250_000, diameter=16.0, K=16.0, seed=2, nrhs=64, ptsmax=100),
     "ellipsoid_4m_x8": dict(kind="ellipsoid", n=4_000_000, K=128.0, seed=3, nrhs=8, ptsmax=400, numC=8),
+    # the reference's WRL meshes sampled at NPW points per wavelength (new_data.m)
+    "f16_k64": dict(kind="wrl", mesh="F16.wrl", K=64.0, NPW=10.0, seed=0, nrhs=1, ptsmax=100),
+    "submarine_k64": dict(kind="wrl", mesh="SubmarineJ.wrl", K=64.0, NPW=10.0, seed=0, nrhs=1, ptsmax=100),
 }
 
 
@@ -90,4 +93,10 @@ def workload_points(name: str) -> np.ndarray:
     w = WORKLOADS[name]
     if w["kind"] == "sphere":
         return sphere(w["n"], w["diameter"], w["seed"])
+    if w["kind"] == "wrl":
+        import os
+
+        from . import wrl
+        pts, tri = wrl.read_wrl(os.path.join(os.path.dirname(os.path.abspath(__file__)), "meshes", w["mesh"]))
+        return wrl.sample(pts, tri, w["K"], w["NPW"], seed=w["seed"])
     return ellipsoid(w["n"], seed=w["seed"])

@@ -265,8 +265,21 @@ def stage_detail(w, st, dd, vd, nrhs, precision, workload):
     roof["launches_per_eval"] = launches
     roof["avg_launch_us"] = t_dom / max(launches, 1) * 1e6
     roof["work"] = work_desc
+    # the same stages one at a time on one stream: each time is the stage's own
+    w.set_timing(2)
+    serial = {}
+    for _ in range(reps):
+        w.eval_device(dd, vd, nrhs)
+        for name, ms, work, kind, prec in w.stage_times():
+            if kind in (ST_FORK, ST_JOIN):
+                continue
+            serial[name] = serial.get(name, 0.0) + ms / reps
+    w.set_timing(False)
+    log("stages alone (ms): " + ", ".join(f"{n}={v:.3f}" for n, v in sorted(serial.items(), key=lambda kv: -kv[1])[:14])
+        + f"; sum {sum(serial.values()):.2f}")
     return {"roofline": roof, "per_kernel_ms": {k: round(v["ms"], 4) for k, v in per.items()},
-            "per_stage_ms": {k: round(v[0], 4) for k, v in per_stage.items()}}
+            "per_stage_ms": {k: round(v[0], 4) for k, v in per_stage.items()},
+            "per_stage_alone_ms": {k: round(v, 4) for k, v in serial.items()}}
 
 
 def main():
@@ -366,6 +379,7 @@ def main():
         "configs": extra,
         "per_kernel_ms": head["per_kernel_ms"],
         "per_stage_ms": head["per_stage_ms"],
+        "per_stage_alone_ms": head["per_stage_alone_ms"],
     }
     print(json.dumps(out), flush=True)
 

@@ -78,6 +78,7 @@ struct ddfmm_ctx {
     int64_t device_bytes = 0;
     bool ready = false;
     bool timing = false;
+    bool serial = false;                       // timing with every stage on one stream
     std::vector<hipEvent_t> ev;
     std::vector<double> stage_ms;
 
@@ -374,7 +375,7 @@ static int run_eval(ddfmm_ctx* c, const double2* den, double2* val, int nrhs, hi
             c->ev.assign(2 * stages.size(), nullptr);
             for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
         }
-        issue_stages(c, nrhs, st, c->timing);
+        issue_stages(c, nrhs, st, c->timing, c->serial);
     }
     HIPCHK(hipEventRecord(c->done_ev, st));
     c->done_recorded = true;
@@ -682,6 +683,7 @@ int ddfmm_read_buffer(ddfmm_ctx* c, int buf, int rhs, int64_t off, int64_t count
 int ddfmm_set_timing(ddfmm_ctx* c, int on) {
     if (!c) return fail(DDFMM_EINVAL, "null argument");
     c->timing = on != 0;
+    c->serial = on == 2;
     return DDFMM_OK;
 }
 

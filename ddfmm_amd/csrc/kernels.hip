@@ -589,6 +589,46 @@ __global__ __launch_bounds__(256) void fft_fwd_kernel(const FftSrc* __restrict__
     }
 }
 
+// Forward transforms of R right-hand sides of one source per block (nrhs > 1): the x and
+// y passes run per right-hand side into its own slab of B, then the z pass runs over
+// (line, rhs) with the right-hand side fastest, so the spectra, stored right-hand side
+// fastest, are written in runs of R consecutive values.
+template <class V, int R>
+__global__ __launch_bounds__(256) void fft_fwd_multi_kernel(const FftSrc* __restrict__ fs,
+                                                            const int* __restrict__ grid, int NL, BufTable bt,
+                                                            int f0, int nf, int nrhs) {
+    __shared__ V sub[216], A[432], B[R][864];
+    const int t = threadIdx.x;
+    const int r0 = blockIdx.y * R, nr = min(R, nrhs - r0);
+    for (int bi = blockIdx.x; bi < nf; bi += gridDim.x) {
+        const FftSrc f = fs[f0 + xcd_item(bi, nf)];
+        for (int r = 0; r < nr; ++r) {
+            const V* in = bufp<V>(bt, BUF_LF_UPEQ) + (int64_t)(r0 + r) * bt.stride[BUF_LF_UPEQ] + f.in_off;
+            __syncthreads();
+            if (t < 216) sub[t] = mk<V>(0, 0);
+            __syncthreads();
+            for (int k = t; k < NL; k += blockDim.x) {
+                const int g = grid[f.grid + k];
+                sub[(g % 12 - 6) + 6 * ((g / 12) % 12 - 6) + 36 * (g / 144 - 6)] = in[k];
+            }
+            __syncthreads();
+            if (t < 36)      // x lines (y', z'): 6 inputs -> 12 outputs, A[z'][y'][kx]
+                dft12_line<V, 6, 12, 6, 0, -1>(sub + 6 * t, 1, A + 12 * t, 1);
+            __syncthreads();
+            if (t < 72)      // y lines (kx, z'): A -> B[r][z'][ky][kx]
+                dft12_line<V, 6, 12, 6, 0, -1>(A + (t % 12) + 72 * (t / 12), 12, B[r] + (t % 12) + 144 * (t / 12),
+                                               12);
+        }
+        __syncthreads();
+        V* out = bufp<V>(bt, BUF_SPEC) + f.spec_off * nrhs + r0;
+        for (int q = t; q < 144 * nr; q += blockDim.x) {   // z lines (kx, ky) x rhs, rhs fastest
+            const int line = q / nr, r = q % nr;
+            dft12_line<V, 6, 12, 6, 0, -1>(B[r] + line, 144, out + (int64_t)line * nrhs + r, 144 * nrhs);
+        }
+        __syncthreads();
+    }
+}
+
 // V list, accumulation: T_t[f] = sum_{(t, s) in V} Tab[offidx(t, s)][f] * S_s[f] over the
 // (2P)^3 frequencies.  A block owns one slice of VSlice<V>::F frequencies (64 B of one
 // spectrum) of one chunk of Morton-consecutive targets.  It stages in LDS the slice of the
@@ -603,91 +643,88 @@ template <class V> struct VSlice {
                                                             // to 72 B: fewer bank conflicts)
 };
 
+constexpr int VL_THREADS = 512;
+
+// Stage the chunk's packed pair words (vpk[k0, k1), a multiple of 8 words, 16-byte
+// aligned) into LDS with int4 loads.
+__device__ __forceinline__ void stage_pairs(const int* __restrict__ vpk, const VChunk& ch, int* pkl, int t) {
+    const int4* src = (const int4*)(vpk + ch.k0);
+    int4* dst = (int4*)pkl;
+    const int n4 = (ch.k1 - ch.k0) >> 2;
+    for (int e = t; e < n4; e += VL_THREADS) dst[e] = src[e];
+}
+
 template <class V>
-__global__ __launch_bounds__(256) void vlist_acc_kernel(const VTarget* __restrict__ vt, const VChunk* __restrict__ vc,
-                                                        const int* __restrict__ vsrc, const int* __restrict__ vpk,
-                                                        const V* __restrict__ opm, BufTable bt, int v0, int c0,
-                                                        int nchunk) {
-    constexpr int G3 = 1728, F = VSlice<V>::F, SE = VSlice<V>::SE, TPP = 256 / F;
+__global__ __launch_bounds__(VL_THREADS) void vlist_acc_kernel(const VTarget* __restrict__ vt,
+                                                               const VChunk* __restrict__ vc,
+                                                               const int* __restrict__ vsrc,
+                                                               const int* __restrict__ vpk,
+                                                               const V* __restrict__ opm, BufTable bt, int v0,
+                                                               int c0, int nchunk) {
+    constexpr int G3 = 1728, F = VSlice<V>::F, SE = VSlice<V>::SE, TPP = VL_THREADS / F;
     __shared__ V tab[343 * SE], src[VCH_SRC * SE];
-    const int rhs = blockIdx.y;
+    __shared__ __align__(16) int pkl[VCH_PK];
     const int t = threadIdx.x, fl = t % F;
     const int item = xcd_item(blockIdx.x, (G3 / F) * nchunk);
     const int f0 = (item / nchunk) * F;
     const VChunk ch = vc[c0 + item % nchunk];
     const V* T = opm + vt[v0].tab_off + f0;
-    const V* spec = bufp<V>(bt, BUF_SPEC) + (int64_t)rhs * bt.stride[BUF_SPEC] + f0;
-    {   // staging: every thread's loads are issued before any is used (two round trips);
-        // row 0 of the source rows is zero
-        constexpr int NT = (343 * F + 255) / 256, NS = (VCH_SRC * F + 255) / 256;
+    const V* spec = bufp<V>(bt, BUF_SPEC) + f0;
+    {   // staging: every thread's loads are issued before any is used; row 0 of the
+        // source rows is zero
+        constexpr int NT = (343 * F + VL_THREADS - 1) / VL_THREADS, NS = (VCH_SRC * F + VL_THREADS - 1) / VL_THREADS;
         const int nu = (ch.u1 - ch.u0) * F;
         V a[NT], b[NS];
         int si[NS];
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
-            const int e = t + 256 * k;
+            const int e = t + VL_THREADS * k;
             si[k] = e < nu ? vsrc[ch.u0 + e / F] : 0;
         }
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
-            const int e = t + 256 * k;
+            const int e = t + VL_THREADS * k;
             a[k] = e < 343 * F ? T[(int64_t)(e / F) * G3 + e % F] : mk<V>(0, 0);
         }
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
-            const int e = t + 256 * k;
+            const int e = t + VL_THREADS * k;
             b[k] = e >= F && e < nu ? spec[(int64_t)si[k] * G3 + e % F] : mk<V>(0, 0);
         }
+        stage_pairs(vpk, ch, pkl, t);
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
-            const int e = t + 256 * k;
+            const int e = t + VL_THREADS * k;
             if (e < 343 * F) tab[(e / F) * SE + e % F] = a[k];
         }
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
-            const int e = t + 256 * k;
+            const int e = t + VL_THREADS * k;
             if (e < nu) src[(e / F) * SE + e % F] = b[k];
         }
     }
     __syncthreads();
     const V* srcl = src + fl;
     const V* tabl = tab + fl;
-    V* out = bufp<V>(bt, BUF_TSPEC) + (int64_t)rhs * bt.stride[BUF_TSPEC] + f0 + fl;
+    V* out = bufp<V>(bt, BUF_TSPEC) + f0 + fl;
     for (int i = ch.t0 + t / F; i < ch.t1; i += TPP) {
         const VTarget tg = vt[i];
         V acc = mk<V>(0, 0);
-        // the target's packed pairs, padded with zero pairs (zero source row) to a multiple
-        // of 8, read F at a time by every lane of its group (same address: a broadcast),
-        // the next F issued before this step's LDS reads
+        // the target's packed pairs (zero pairs pad to a multiple of 8; row 0 is zero),
+        // read from LDS by every lane of its group (one address: a broadcast)
         const int n = (tg.p1 - tg.p0 + 7) & ~7;
-        const int4* pk = (const int4*)(vpk + tg.k0);
-        auto step = [&](const int4* q4) {
+        const int4* pk = (const int4*)(pkl + (tg.k0 - ch.k0));
+        for (int j = 0; j < n; j += F) {
             V sv[F], tv[F];
 #pragma unroll
             for (int u = 0; u < F; ++u) {
-                const int4 q = q4[u / 4];
+                const int4 q = pk[(j + u) / 4];
                 const int w = (u % 4 == 0) ? q.x : (u % 4 == 1) ? q.y : (u % 4 == 2) ? q.z : q.w;
                 sv[u] = srcl[(w & 0xffff) * SE];
                 tv[u] = tabl[(w >> 16) * SE];
             }
 #pragma unroll
             for (int u = 0; u < F; ++u) cfma(acc, sv[u], tv[u]);
-        };
-        int4 qa[F / 4], qb[F / 4];
-#pragma unroll
-        for (int c = 0; c < F / 4; ++c) qa[c] = pk[c];
-        for (int j = 0; j < n; j += 2 * F) {     // two steps per pass, the next F pairs in flight
-            if (j + F < n) {
-#pragma unroll
-                for (int c = 0; c < F / 4; ++c) qb[c] = pk[(j + F) / 4 + c];
-            }
-            step(qa);
-            if (j + F >= n) break;
-            if (j + 2 * F < n) {
-#pragma unroll
-                for (int c = 0; c < F / 4; ++c) qa[c] = pk[(j + 2 * F) / 4 + c];
-            }
-            step(qb);
         }
         out[(int64_t)(i - v0) * G3] = acc;
     }
@@ -706,7 +743,7 @@ __global__ __launch_bounds__(256) void vlist_acc_kernel(const VTarget* __restric
 template <class V> struct VMulti { static constexpr int R2 = 16 / (int)sizeof(V); };   // float2: 2, double2: 1
 
 template <class V, int F, int R1>
-__global__ __launch_bounds__(256) void vlist_acc_multi_kernel(const VTarget* __restrict__ vt,
+__global__ __launch_bounds__(VL_THREADS) void vlist_acc_multi_kernel(const VTarget* __restrict__ vt,
                                                               const VChunk* __restrict__ vc,
                                                               const int* __restrict__ vsrc,
                                                               const int* __restrict__ vpk,
@@ -717,6 +754,7 @@ __global__ __launch_bounds__(256) void vlist_acc_multi_kernel(const VTarget* __r
     static_assert(ROW * (int)sizeof(V) == 128, "128-byte source rows");
     __shared__ V tab[343 * F];
     __shared__ __align__(16) V src[VCH_SRC * ROW];
+    __shared__ __align__(16) int pkl[VCH_PK];
     const int t = threadIdx.x, l = t & 7;
     const int fl = l / R1, rq = (l % R1) * R2;
     const int item = xcd_item(blockIdx.x, NSL * nchunk);
@@ -726,31 +764,32 @@ __global__ __launch_bounds__(256) void vlist_acc_multi_kernel(const VTarget* __r
     const V* T0 = opm + vt[v0].tab_off + f0;
     const V* spec = bufp<V>(bt, BUF_SPEC);
     {   // staging, all loads of a thread issued before its LDS writes; row 0 is zero
-        constexpr int NT = (343 * F + 255) / 256, NS = (VCH_SRC * ROW + 255) / 256;
+        constexpr int NT = (343 * F + VL_THREADS - 1) / VL_THREADS, NS = (VCH_SRC * ROW + VL_THREADS - 1) / VL_THREADS;
         const int nu = (ch.u1 - ch.u0) * ROW;
         V a[NT], b[NS];
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
-            const int e = t + 256 * k;
+            const int e = t + VL_THREADS * k;
             a[k] = e < 343 * F ? T0[(int64_t)(e / F) * G3 + e % F] : mk<V>(0, 0);
         }
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
-            const int e = t + 256 * k;
+            const int e = t + VL_THREADS * k;
             const int u = e / ROW, w = e % ROW, f = w / RC, r = rg + w % RC;
             V v = mk<V>(0, 0);
             if (e >= ROW && e < nu && r < nrhs)
                 v = spec[((int64_t)vsrc[ch.u0 + u] * G3 + f0 + f) * nrhs + r];
             b[k] = v;
         }
+        stage_pairs(vpk, ch, pkl, t);
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
-            const int e = t + 256 * k;
+            const int e = t + VL_THREADS * k;
             if (e < 343 * F) tab[e] = a[k];
         }
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
-            const int e = t + 256 * k;
+            const int e = t + VL_THREADS * k;
             if (e < nu) src[e] = b[k];
         }
     }
@@ -759,13 +798,13 @@ __global__ __launch_bounds__(256) void vlist_acc_multi_kernel(const VTarget* __r
     const V* srcl = src + l * R2;
     V* out = bufp<V>(bt, BUF_TSPEC);
     const int64_t os = bt.stride[BUF_TSPEC];
-    for (int i = ch.t0 + t / 8; i < ch.t1; i += 32) {
+    for (int i = ch.t0 + t / 8; i < ch.t1; i += VL_THREADS / 8) {
         const VTarget tg = vt[i];
         V acc[R2];
 #pragma unroll
         for (int q = 0; q < R2; ++q) acc[q] = mk<V>(0, 0);
         const int n = (tg.p1 - tg.p0 + 7) & ~7;
-        const int4* pk = (const int4*)(vpk + tg.k0);
+        const int4* pk = (const int4*)(pkl + (tg.k0 - ch.k0));
         auto step = [&](const int4* q4) {
             V tv[8], sv[8][R2];
 #pragma unroll
@@ -787,22 +826,7 @@ __global__ __launch_bounds__(256) void vlist_acc_multi_kernel(const VTarget* __r
 #pragma unroll
                 for (int q = 0; q < R2; ++q) cfma(acc[q], sv[u][q], tv[u]);   // operand order of the 1-RHS kernel
         };
-        int4 qa[2], qb[2];
-        qa[0] = pk[0];
-        qa[1] = pk[1];
-        for (int j = 0; j < n; j += 16) {      // two steps of 8 pairs per pass, the next 8 in flight
-            if (j + 8 < n) {
-                qb[0] = pk[(j + 8) / 4];
-                qb[1] = pk[(j + 8) / 4 + 1];
-            }
-            step(qa);
-            if (j + 8 >= n) break;
-            if (j + 16 < n) {
-                qa[0] = pk[(j + 16) / 4];
-                qa[1] = pk[(j + 16) / 4 + 1];
-            }
-            step(qb);
-        }
+        for (int j = 0; j < n; j += 8) step(pk + j / 4);
 #pragma unroll
         for (int q = 0; q < R2; ++q) {
             const int r = rg + rq + q;
@@ -1081,8 +1105,14 @@ hipError_t launch_stage_t(const DevPlan& d, const Stage& s, int nrhs, hipStream_
         break;
     case ST_FFTFWD:
         if (d.P != 6) return hipErrorInvalidValue;
-        fft_fwd_kernel<V><<<dim3(s.stream ? std::min(n, AUX_BLOCKS) : n, nrhs), 256, 0, st>>>(d.fsrc, d.grid, d.NL,
-                                                                                              d.bt, s.begin, n, nrhs);
+        if (nrhs == 1) {
+            fft_fwd_kernel<V><<<dim3(s.stream ? std::min(n, AUX_BLOCKS) : n, 1), 256, 0, st>>>(d.fsrc, d.grid, d.NL,
+                                                                                           d.bt, s.begin, n, 1);
+        } else {    // 8 (FP32) or 4 (FP64) right-hand sides per block: 55 KB of LDS for B
+            constexpr int R = sizeof(V) == 8 ? 8 : 4;
+            fft_fwd_multi_kernel<V, R><<<dim3(n, (nrhs + R - 1) / R), 256, 0, st>>>(d.fsrc, d.grid, d.NL, d.bt,
+                                                                                 s.begin, n, nrhs);
+        }
         break;
     case ST_VLIST: {
         if (d.P != 6) return hipErrorInvalidValue;
@@ -1091,19 +1121,19 @@ hipError_t launch_stage_t(const DevPlan& d, const Stage& s, int nrhs, hipStream_
         const V* opm = d.mixed ? (const V*)d.opmat32 : (const V*)d.opmat;
         if (nrhs == 1) {
             constexpr int F = VSlice<V>::F;
-            vlist_acc_kernel<V><<<dim3((1728 / F) * nchunk, 1), 256, 0, st>>>(d.vtrg, d.vchunk, d.vsrc, d.vpk, opm,
+            vlist_acc_kernel<V><<<dim3((1728 / F) * nchunk, 1), VL_THREADS, 0, st>>>(d.vtrg, d.vchunk, d.vsrc, d.vpk, opm,
                                                                               d.bt, s.begin, s.buf, nchunk);
         } else {
             // right-hand sides per block: 16 / 8 / 4 (FP32) or 8 / 4 / 2 (FP64) by nrhs
             constexpr int R2 = VMulti<V>::R2;
             if (nrhs >= 8 * R2) {
-                vlist_acc_multi_kernel<V, 1, 8><<<dim3(1728 * nchunk, (nrhs + 8 * R2 - 1) / (8 * R2)), 256, 0, st>>>(
+                vlist_acc_multi_kernel<V, 1, 8><<<dim3(1728 * nchunk, (nrhs + 8 * R2 - 1) / (8 * R2)), VL_THREADS, 0, st>>>(
                     d.vtrg, d.vchunk, d.vsrc, d.vpk, opm, d.bt, s.begin, s.buf, nchunk, nrhs);
             } else if (nrhs >= 4 * R2) {
-                vlist_acc_multi_kernel<V, 2, 4><<<dim3(864 * nchunk, (nrhs + 4 * R2 - 1) / (4 * R2)), 256, 0, st>>>(
+                vlist_acc_multi_kernel<V, 2, 4><<<dim3(864 * nchunk, (nrhs + 4 * R2 - 1) / (4 * R2)), VL_THREADS, 0, st>>>(
                     d.vtrg, d.vchunk, d.vsrc, d.vpk, opm, d.bt, s.begin, s.buf, nchunk, nrhs);
             } else {
-                vlist_acc_multi_kernel<V, 4, 2><<<dim3(432 * nchunk, (nrhs + 2 * R2 - 1) / (2 * R2)), 256, 0, st>>>(
+                vlist_acc_multi_kernel<V, 4, 2><<<dim3(432 * nchunk, (nrhs + 2 * R2 - 1) / (2 * R2)), VL_THREADS, 0, st>>>(
                     d.vtrg, d.vchunk, d.vsrc, d.vpk, opm, d.bt, s.begin, s.buf, nchunk, nrhs);
             }
         }

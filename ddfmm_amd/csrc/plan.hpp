@@ -101,12 +101,16 @@ struct VPair {
 // union vsrc[u0, u1) of at most VCH_SRC rows (row 0 a zero row, then source spectrum
 // indices), are staged in LDS together.  The device reads each pair packed as
 // local row | (offidx << 16); a zero entry (row 0) pads.
+// The chunk's packed pairs are the contiguous range vpk[k0, k1) (its targets' lists in
+// target order), at most VCH_PK words, staged in LDS with the sources.
 struct VChunk {
     int t0, t1;
     int u0, u1;
+    int k0, k1;
 };
 constexpr int VCH_SRC = 400;
 constexpr int VCH_TRG = 256;
+constexpr int VCH_PK = 8192;
 
 // segmented reduction: out[out_off + m] (=|+=) sum_{j < cnt} BUF_RED[red_off + j * M + m]
 // (the contributions of one target are adjacent, in a fixed order: deterministic sums)

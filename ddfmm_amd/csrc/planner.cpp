@@ -383,12 +383,14 @@ struct Builder {
                     need(ch.t0 == next && ch.t0 < ch.t1 && ch.t1 <= st.end, "vlist chunk targets");
                     need(ch.u0 >= 0 && ch.u0 <= ch.u1 && ch.u1 - ch.u0 <= VCH_SRC &&
                              ch.u1 <= (int)pl.vsrc.size(), "vlist chunk sources");
+                    need(ch.k0 >= 0 && ch.k0 % 8 == 0 && ch.k0 <= ch.k1 && ch.k1 - ch.k0 <= VCH_PK &&
+                             ch.k1 <= (int)pl.vpk.size(), "vlist chunk pairs");
                     for (int u = ch.u0; u < ch.u1; ++u)
                         need(pl.vsrc[u] >= 0 && (int64_t)pl.vsrc[u] * G3 + G3 <= pl.bufsize[BUF_SPEC], "vlist chunk source");
                     for (int t = ch.t0; t < ch.t1; ++t) {
                         const VTarget& v = pl.vtrg[t];
                         const int n8 = (v.p1 - v.p0 + 7) & ~7;
-                        need(v.k0 >= 0 && v.k0 % 8 == 0 && v.k0 + n8 <= (int)pl.vpk.size(), "vlist packed range");
+                        need(v.k0 >= ch.k0 && v.k0 % 8 == 0 && v.k0 + n8 <= ch.k1, "vlist packed range");
                         for (int j = 0; j < n8; ++j) {
                             const int w = pl.vpk[v.k0 + j], l = w & 0xffff, o = w >> 16;
                             if (j >= v.p1 - v.p0) {
@@ -1279,14 +1281,19 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
                 pl.vchunk.push_back(ch);
             };
             open(v0);
+            int npk = 0;                                   // padded pair words of the open chunk
             for (int i = v0; i < (int)pl.vtrg.size(); ++i) {
                 const VTarget& v = pl.vtrg[i];
                 int add = 0;
                 for (int q = v.p0; q < v.p1; ++q) add += !loc.count(pl.vpair[q].spec_off / G3);
-                if (i > ch.t0 && (1 + (int)loc.size() + add > VCH_SRC || i - ch.t0 >= VCH_TRG)) {
+                const int n8 = (v.p1 - v.p0 + 7) & ~7;
+                if (n8 > VCH_PK) throw Error("internal: V list longer than a chunk's pair buffer");
+                if (i > ch.t0 && (1 + (int)loc.size() + add > VCH_SRC || i - ch.t0 >= VCH_TRG || npk + n8 > VCH_PK)) {
                     close(i);
                     open(i);
+                    npk = 0;
                 }
+                npk += n8;
                 for (int q = v.p0; q < v.p1; ++q) {
                     const int64_t s = pl.vpair[q].spec_off / G3;
                     auto it = loc.find(s);
@@ -1314,6 +1321,12 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
             v.k0 = (int)pl.vpk.size();
             for (int q = v.p0; q < v.p1; ++q) pl.vpk.push_back(lpk[q]);
             while (pl.vpk.size() % 8) pl.vpk.push_back(0);
+        }
+        for (int c = c0; c < (int)pl.vchunk.size(); ++c) {
+            VChunk& ch = pl.vchunk[c];
+            const VTarget& last = pl.vtrg[ch.t1 - 1];
+            ch.k0 = pl.vtrg[ch.t0].k0;
+            ch.k1 = last.k0 + ((last.p1 - last.p0 + 7) & ~7);
         }
         stage(ST_VLIST, v0, (int)pl.vtrg.size(), "lf_vlist", c0, (int64_t)pl.vchunk.size() - c0);
         // X list into the check potential

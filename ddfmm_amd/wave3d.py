@@ -197,7 +197,9 @@ class Wave3d:
         check(lib().ddfmm_read_buffer(self._ctx, buf, rhs, off, count, ptr(out)))
         return out
 
-    def set_timing(self, on: bool = True):
+    def set_timing(self, on=True):
+        """Per-stage HIP-event timing of the following evaluations (ddfmm_set_timing);
+        on = 2 issues every stage on one stream, so that each time is the stage's own."""
         check(lib().ddfmm_set_timing(self._ctx, int(on)))
 
     def autotune(self, nrhs: int = 1, reps: int = 2) -> dict:

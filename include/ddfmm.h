@@ -152,7 +152,9 @@ DDFMM_API int ddfmm_read_buffer(ddfmm_ctx* ctx, int buf, int rhs, int64_t off, i
 /* Per-stage timing: enable, then read, per stage of the last eval, its name, milliseconds
    (HIP events on the evaluation stream), algorithmic work per RHS (GEMM: useful complex
    flops 8*M*K*N; kernel apply: target-source interactions; FFT: spectra; V list: pairs)
-   and kind (stage kind + 16 * arithmetic class, 0 translation / 1 FP64 solve). */
+   and kind (stage kind + 16 * arithmetic class, 0 translation / 1 FP64 solve).
+   on = 2 issues every stage on the evaluation stream (no overlap between stages), so
+   that each time is the stage's own. */
 DDFMM_API int ddfmm_set_timing(ddfmm_ctx* ctx, int on);
 DDFMM_API int ddfmm_stage_times(ddfmm_ctx* ctx, int cap, char* names /* cap x 24 */, double* ms, double* work,
                       int* kinds, int* count);

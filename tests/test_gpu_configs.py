@@ -170,3 +170,17 @@ def test_fp64_fused_vs_oracle_and_direct(oplib, ops, N, K, seed):
     assert e < 1e-7 and ed < FMM_TOL
     assert np.array_equal(w.eval(den[2]), val[2])
     w.close()
+
+
+@pytest.mark.parametrize("workload", ["f16_k64", "submarine_k64"])
+def test_wrl_meshes(oplib, workload):
+    """The reference's F16 and submarine meshes (data/*.wrl) sampled at 10 points per
+    wavelength in a K = 64 domain (new_data.m), mixed precision, within 1e-5 of the
+    FP64 direct sum."""
+    w, pos, wl = _wave(workload, oplib, MIXED)
+    den = geometry.densities(len(pos), wl["seed"])
+    val = w.eval(den)
+    e = _check(pos, val, den, wl["seed"])
+    print(f"{workload}: {len(pos)} points, mixed vs direct {e:.3e}", flush=True)
+    assert e < FMM_TOL
+    w.close()

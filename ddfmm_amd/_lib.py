@@ -13,7 +13,8 @@ EXPORTS = ["ddfmm_last_error", "ddfmm_create", "ddfmm_destroy", "ddfmm_setup", "
            "ddfmm_eval_device", "ddfmm_direct_sum", "ddfmm_kernel_matrix", "ddfmm_get_stats",
            "ddfmm_dump_lists", "ddfmm_read_buffer", "ddfmm_set_timing", "ddfmm_stage_times",
            "ddfmm_stage_exec", "ddfmm_set_graph", "ddfmm_mlib_load", "ddfmm_mlib_free", "ddfmm_mlib_widths",
-           "ddfmm_setup_mlib", "ddfmm_autotune", "ddfmm_set_tiles", "ddfmm_get_tiles", "ddfmm_direct_sum_noalloc"]
+           "ddfmm_setup_mlib", "ddfmm_autotune", "ddfmm_set_tiles", "ddfmm_get_tiles", "ddfmm_direct_sum_noalloc",
+           "ddfmm_set_hbm_budget", "ddfmm_memory"]
 
 
 class Params(ctypes.Structure):
@@ -75,6 +76,8 @@ def lib():
                                  ctypes.POINTER(cd)]
     L.ddfmm_set_tiles.argtypes = [vp, ci, ctypes.POINTER(ci)]
     L.ddfmm_get_tiles.argtypes = [vp, ci, ctypes.POINTER(ci), ctypes.c_char_p, ctypes.POINTER(ci)]
+    L.ddfmm_set_hbm_budget.argtypes = [vp, i64]
+    L.ddfmm_memory.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(ci)]
     L.ddfmm_stage_times.argtypes = [vp, ci, ctypes.c_char_p, ctypes.POINTER(cd), ctypes.POINTER(cd),
                                      ctypes.POINTER(ci), ctypes.POINTER(ci)]
     L.ddfmm_stage_exec.argtypes = [vp, ci, ctypes.POINTER(cd), ctypes.POINTER(ci)]

@@ -72,6 +72,9 @@ struct ddfmm_ctx {
     size_t tiles_bytes = 0;                    // device bytes of the GEMM tile table
     Params prm;
     int nrhs_max = 1, precision = 0;
+    int64_t hbm_budget = 0;                    // bytes; 0 = no budget
+    int slab = 1;                              // right-hand sides per pass (work buffers sized for it)
+    int64_t fixed_bytes = 0, rhs_bytes = 0, setup_transient = 0;
     HostPlan hp;
     DevPlan dp;
     std::vector<void*> allocs;
@@ -111,6 +114,31 @@ struct ddfmm_ctx {
         ready = false;
     }
 };
+
+// Device bytes of a set-up context: `fixed` for the point, operator and descriptor
+// tables (kept for the context's life), `per_rhs` for the work buffers and the staging
+// of one right-hand side, `transient` for the generator tables that live only during
+// setup (freed before the work buffers are allocated).
+template <class T>
+static int64_t vbytes(const std::vector<T>& v) { return (int64_t)v.size() * (int64_t)sizeof(T); }
+static void plan_bytes(const HostPlan& h, int precision, int64_t& fixed, int64_t& per_rhs, int64_t& transient) {
+    const bool mixed = precision == DDFMM_MIXED;
+    fixed = vbytes(h.px) + vbytes(h.py) + vbytes(h.pz) + vbytes(h.perm) + vbytes(h.grid) +
+            h.opmat_total * (int64_t)sizeof(double2) + (mixed ? h.opmat_total * (int64_t)sizeof(float2) : 0) +
+            vbytes(h.ktasks) + vbytes(h.ksrcs) + vbytes(h.groups) + vbytes(h.tiles) + vbytes(h.colB) +
+            vbytes(h.colC) + vbytes(h.colA) + vbytes(h.fsrc) + vbytes(h.vtrg) + vbytes(h.vchunk) +
+            vbytes(h.vsrc) + vbytes(h.vpk) + vbytes(h.rtrg) + (int64_t)sizeof(IoPtrs);
+    const int64_t esz = mixed ? (int64_t)sizeof(float2) : (int64_t)sizeof(double2);
+    per_rhs = 2 * h.N * (int64_t)sizeof(double2);
+    for (int b = 0; b < BUF_COUNT; ++b) per_rhs += h.bufsize[b] * esz;
+    transient = vbytes(h.genpts) + vbytes(h.kgen);
+}
+
+static std::string gib(int64_t b) {
+    char t[32];
+    std::snprintf(t, sizeof(t), "%.3f GiB", (double)b / (1 << 30));
+    return t;
+}
 
 extern "C" {
 
@@ -236,6 +264,21 @@ static int setup_common(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_
     } catch (const std::exception& e) {
         return fail(DDFMM_ESETUP, e.what());
     }
+    // HBM budget: the work buffers hold `slab` right-hand sides; an evaluation of more
+    // runs in passes of at most `slab` (each right-hand side is independent, so the
+    // results are the same bits as in one pass).
+    plan_bytes(c->hp, c->precision, c->fixed_bytes, c->rhs_bytes, c->setup_transient);
+    c->slab = c->nrhs_max;
+    if (c->hbm_budget > 0) {
+        const int64_t need = c->fixed_bytes + std::max(c->rhs_bytes, c->setup_transient);
+        if (need > c->hbm_budget) {
+            c->ready = false;
+            return fail(DDFMM_ENOMEM, "HBM budget " + gib(c->hbm_budget) + " below the minimum " + gib(need) +
+                                          " (tables " + gib(c->fixed_bytes) + ", one right-hand side " +
+                                          gib(c->rhs_bytes) + ")");
+        }
+        c->slab = (int)std::min<int64_t>(c->nrhs_max, (c->hbm_budget - c->fixed_bytes) / c->rhs_bytes);
+    }
     if (c->device < 0) {
         c->ready = true;
         return DDFMM_OK;
@@ -246,6 +289,10 @@ static int setup_common(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_
         d = DevPlan();
         d.N = h.N; d.P = h.P; d.NL = h.NL; d.ktype = c->prm.ktype;
         d.mixed = c->precision == DDFMM_MIXED ? 1 : 0;     // FP64 and FP64_FUSED: FP64 arithmetic
+        // the kernel-apply flavour is fixed per context (each right-hand side then has the
+        // same bits alone and in any batch): the matrix-core kernel from four right-hand
+        // sides up, the vector-ALU kernel below
+        d.kmfma = d.mixed && c->nrhs_max >= 4 ? 1 : 0;
         const int64_t esz = d.mixed ? (int64_t)sizeof(float2) : (int64_t)sizeof(double2);
         d.px = c->upload(h.px); d.py = c->upload(h.py); d.pz = c->upload(h.pz);
         d.perm = c->upload(h.perm);
@@ -260,13 +307,6 @@ static int setup_common(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_
         d.fsrc = c->upload(h.fsrc); d.vtrg = c->upload(h.vtrg);
         d.vchunk = c->upload(h.vchunk); d.vsrc = c->upload(h.vsrc); d.vpk = c->upload(h.vpk);
         d.rtrg = c->upload(h.rtrg);
-        for (int b = 0; b < BUF_COUNT; ++b) {
-            d.bt.stride[b] = h.bufsize[b];
-            d.bt.p[b] = esz == 8 ? (void*)c->alloc<float2>(h.bufsize[b] * c->nrhs_max)
-                                 : (void*)c->alloc<double2>(h.bufsize[b] * c->nrhs_max);
-        }
-        d.den_io = c->alloc<double2>(h.N * c->nrhs_max);
-        d.val_io = c->alloc<double2>(h.N * c->nrhs_max);
         d.io = c->alloc<IoPtrs>(1);
         // translation matrices generated on the device (temporary generator tables)
         if (!h.kgen.empty()) {
@@ -293,6 +333,14 @@ static int setup_common(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_
             HIPCHK(launch_to_f32(d.opmat, d.opmat32, h.opmat_total, c->stream));
             HIPCHK(hipStreamSynchronize(c->stream));
         }
+        // work buffers, after the setup-time tables are gone
+        for (int b = 0; b < BUF_COUNT; ++b) {
+            d.bt.stride[b] = h.bufsize[b];
+            d.bt.p[b] = esz == 8 ? (void*)c->alloc<float2>(h.bufsize[b] * c->slab)
+                                 : (void*)c->alloc<double2>(h.bufsize[b] * c->slab);
+        }
+        d.den_io = c->alloc<double2>(h.N * c->slab);
+        d.val_io = c->alloc<double2>(h.N * c->slab);
         // host copies no longer needed on the device path
         std::vector<cpx>().swap(h.opmat);
         std::vector<double>().swap(h.genpts);
@@ -398,11 +446,15 @@ int ddfmm_eval(ddfmm_ctx* c, const double* den, double* val, int nrhs) {
     if (nrhs < 1 || nrhs > c->nrhs_max) return fail(DDFMM_EINVAL, "nrhs out of range [1, nrhs_max]");
     try {
         HIPCHK(hipSetDevice(c->device));
-        size_t bytes = (size_t)c->hp.N * nrhs * sizeof(double2);
-        if (c->done_recorded) HIPCHK(hipStreamWaitEvent(c->stream, c->done_ev, 0));
-        HIPCHK(hipMemcpyAsync(c->dp.den_io, den, bytes, hipMemcpyHostToDevice, c->stream));
-        run_eval(c, c->dp.den_io, c->dp.val_io, nrhs, c->stream);
-        HIPCHK(hipMemcpyAsync(val, c->dp.val_io, bytes, hipMemcpyDeviceToHost, c->stream));
+        for (int r0 = 0; r0 < nrhs; r0 += c->slab) {
+            const int nr = std::min(c->slab, nrhs - r0);
+            const size_t bytes = (size_t)c->hp.N * nr * sizeof(double2);
+            const size_t off = (size_t)c->hp.N * r0 * 2;
+            if (c->done_recorded) HIPCHK(hipStreamWaitEvent(c->stream, c->done_ev, 0));
+            HIPCHK(hipMemcpyAsync(c->dp.den_io, den + off, bytes, hipMemcpyHostToDevice, c->stream));
+            run_eval(c, c->dp.den_io, c->dp.val_io, nr, c->stream);
+            HIPCHK(hipMemcpyAsync(val + off, c->dp.val_io, bytes, hipMemcpyDeviceToHost, c->stream));
+        }
         HIPCHK(hipStreamSynchronize(c->stream));
     } catch (const std::exception& e) {
         return fail(DDFMM_EDEVICE, e.what());
@@ -417,7 +469,9 @@ int ddfmm_eval_device(ddfmm_ctx* c, const void* den_dev, void* val_dev, int nrhs
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
     try {
         HIPCHK(hipSetDevice(c->device));
-        run_eval(c, (const double2*)den_dev, (double2*)val_dev, nrhs, st);
+        for (int r0 = 0; r0 < nrhs; r0 += c->slab)
+            run_eval(c, (const double2*)den_dev + c->hp.N * r0, (double2*)val_dev + c->hp.N * r0,
+                     std::min(c->slab, nrhs - r0), st);
     } catch (const std::exception& e) {
         return fail(DDFMM_EDEVICE, e.what());
     }
@@ -447,8 +501,10 @@ static void upload_tiles(ddfmm_ctx* c) {
         c->allocs.erase(it);
         c->device_bytes -= (int64_t)c->tiles_bytes;
     }
+    c->fixed_bytes -= (int64_t)c->tiles_bytes;
     c->dp.tiles = c->upload(c->hp.tiles);
     c->tiles_bytes = c->hp.tiles.size() * sizeof(GTile);
+    c->fixed_bytes += (int64_t)c->tiles_bytes;
     c->drop_graphs();
     c->ev.clear();
     c->stage_ms.clear();
@@ -486,6 +542,7 @@ int ddfmm_autotune(ddfmm_ctx* c, int nrhs, int reps, int cap, int* mbmax, int* c
     if (!c || !count) return fail(DDFMM_EINVAL, "null argument");
     if (!c->ready || c->device < 0) return fail(DDFMM_ESTATE, "context not set up on a device");
     if (nrhs < 1 || nrhs > c->nrhs_max) return fail(DDFMM_EINVAL, "nrhs out of range [1, nrhs_max]");
+    nrhs = std::min(nrhs, c->slab);
     reps = std::max(1, reps);
     const int nl = (int)c->hp.lgemm.size();
     *count = nl;
@@ -658,7 +715,7 @@ int ddfmm_dump_lists(ddfmm_ctx* c, char* buf, int64_t cap, int64_t* len) {
 int ddfmm_read_buffer(ddfmm_ctx* c, int buf, int rhs, int64_t off, int64_t count, double* out) {
     if (!c || !out) return fail(DDFMM_EINVAL, "null argument");
     if (!c->ready || c->device < 0) return fail(DDFMM_ESTATE, "context not set up on a device");
-    if (buf < 0 || buf >= BUF_COUNT || rhs < 0 || rhs >= c->nrhs_max || off < 0 || count < 0 ||
+    if (buf < 0 || buf >= BUF_COUNT || rhs < 0 || rhs >= c->slab || off < 0 || count < 0 ||
         off + count > c->hp.bufsize[buf])
         return fail(DDFMM_EINVAL, "buffer range out of bounds");
     if (count == 0) return DDFMM_OK;
@@ -677,6 +734,23 @@ int ddfmm_read_buffer(ddfmm_ctx* c, int buf, int rhs, int64_t off, int64_t count
     } catch (const std::exception& e) {
         return fail(DDFMM_EDEVICE, e.what());
     }
+    return DDFMM_OK;
+}
+
+int ddfmm_set_hbm_budget(ddfmm_ctx* c, int64_t bytes) {
+    if (!c) return fail(DDFMM_EINVAL, "null argument");
+    if (bytes < 0) return fail(DDFMM_EINVAL, "negative HBM budget");
+    c->hbm_budget = bytes;
+    return DDFMM_OK;
+}
+
+int ddfmm_memory(ddfmm_ctx* c, int64_t* fixed_bytes, int64_t* per_rhs_bytes, int64_t* setup_bytes, int* slab) {
+    if (!c) return fail(DDFMM_EINVAL, "null argument");
+    if (!c->ready) return fail(DDFMM_ESTATE, "context not set up");
+    if (fixed_bytes) *fixed_bytes = c->fixed_bytes;
+    if (per_rhs_bytes) *per_rhs_bytes = c->rhs_bytes;
+    if (setup_bytes) *setup_bytes = c->setup_transient;
+    if (slab) *slab = c->slab;
     return DDFMM_OK;
 }
 

@@ -25,6 +25,7 @@ struct DevPlan {
     int64_t N = 0;
     int P = 0, NL = 0, ktype = 0;
     int mixed = 0;                 // 1: FP32 vectors/translations, FP64 solves
+    int kmfma = 0;                 // mixed and nrhs_max >= 4: kernel applies on the matrix cores
     double *px = nullptr, *py = nullptr, *pz = nullptr;
     int* perm = nullptr;
     int* grid = nullptr;

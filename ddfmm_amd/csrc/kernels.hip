@@ -26,6 +26,12 @@
 
 #include "device.hpp"
 
+// No implicit FP contraction: a + b * c is fused only where the code says fma.  Kernels
+// that serve different right-hand-side counts (1-RHS and multi-RHS instances) then round
+// every operation the same way, which keeps each right-hand side's bits independent of
+// the batch it is evaluated in.
+#pragma clang fp contract(off)
+
 namespace ddfmm {
 
 namespace {
@@ -50,7 +56,7 @@ __device__ __forceinline__ void cfma(V& acc, V a, V b) {
 
 // FP64 G(r), k = 2*pi: e^{ikr}/r (type 0) or e^{ikr} (type 1); r^2 < 1e-16 => r := 1.
 __device__ __forceinline__ double2 green64(double dx, double dy, double dz, int ktype) {
-    double r2 = dx * dx + dy * dy + dz * dz;
+    double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
     r2 = r2 < 1e-16 ? 1.0 : r2;
     double r = sqrt(r2);
     double s, c;
@@ -66,11 +72,11 @@ __device__ __forceinline__ double2 green64(double dx, double dy, double dz, int 
 // v_cos_f32 on the phase r - rint(r) in revolutions (three transcendental operations).
 __device__ __forceinline__ float2 green32(double dx, double dy, double dz, int ktype) {
     float fx = (float)dx, fy = (float)dy, fz = (float)dz;
-    float r2 = fx * fx + fy * fy + fz * fz;
+    float r2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
     r2 = r2 < 1e-16f ? 1.0f : r2;
     const float ir = __builtin_amdgcn_rsqf(r2);
-    const float r = r2 * ir;
-    float t = r - __builtin_rintf(r);
+    // phase r - rint(r) with the product r2 * ir unrounded
+    const float t = fmaf(r2, ir, -__builtin_rintf(r2 * ir));
     float s = __builtin_amdgcn_sinf(t), c = __builtin_amdgcn_cosf(t);
     if (ktype == 0) return make_float2(c * ir, s * ir);
     return make_float2(c, s);
@@ -189,6 +195,211 @@ __global__ __launch_bounds__(KA_THREADS) void kapply_kernel(
                     v.x += acc[q].x;
                     v.y += acc[q].y;
                     *o = v;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    }
+}
+
+// Kernel apply on the matrix cores (mixed precision): out_q(t) = sum_s G(t, s) v_q(s) for
+// 8 * NB right-hand sides, per block of 16 targets x 4 sources as two real products on
+// v_mfma_f32_16x16x4f32 whose 16 columns are n = (rhs q, re/im):
+//   C += Re G . B1,   B1[s][(q, 0)] = Re v_q(s),   B1[s][(q, 1)] = Im v_q(s)
+//   C += Im G . B2,   B2[s][(q, 0)] = -Im v_q(s),  B2[s][(q, 1)] = Re v_q(s)
+// so that C[t][(q, 0)] = Re out_q(t) and C[t][(q, 1)] = Im out_q(t).  Lane i + 16 k holds
+// A[i][k] and B[k][i]: it evaluates G for its own pair (target i, source k) on the VALU
+// (FP64 differences, FP32 G, the formula of the VALU path) and feeds Re G and Im G to the
+// two products unchanged; its B entries come from the density of source k for right-hand
+// side i / 2.  A block owns one task; targets go in passes of 256 (16 row tiles of 16),
+// the four waves split the row tiles (up to 4 each, every source step serving all of
+// them) and, for up to 8 row tiles, the sources, with a fixed-order reduction through
+// LDS.  Every column's k order is the same for any right-hand-side count, so each
+// right-hand side keeps its bits in any batch.
+constexpr int KM_ST = 256;    // sources per LDS tile
+
+typedef float kacc4 __attribute__((ext_vector_type(4)));
+
+// One wave's pass over the source groups gs = g0, g0 + gstep, ... < ng of an LDS tile for
+// its NM row tiles.  The next group's coordinates and densities are read from LDS while
+// the current group is evaluated.
+template <int NM, int NB, int KT>
+__device__ __forceinline__ void kmfma_steps(kacc4 (&acc)[4][NB], const double (&tx)[4], const double (&ty)[4],
+                                            const double (&tz)[4], const double* sx, const double* sy,
+                                            const double* sz, const float2 (*sv)[8 * NB + 1], int g0, int gstep,
+                                            int ng, int g, int li) {
+    const bool imcol = li & 1;
+    if (g0 >= ng) return;
+    int j = 4 * g0 + g;
+    double xs = sx[j], ys = sy[j], zs = sz[j];
+    float2 v[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) v[c] = sv[j][8 * c + (li >> 1)];
+    for (int gs = g0; gs < ng; gs += gstep) {
+        const int jn = gs + gstep < ng ? 4 * (gs + gstep) + g : j;
+        const double xn = sx[jn], yn = sy[jn], zn = sz[jn];
+        float2 vn[NB];
+#pragma unroll
+        for (int c = 0; c < NB; ++c) vn[c] = sv[jn][8 * c + (li >> 1)];
+        float b1[NB], b2[NB];
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            b1[c] = imcol ? v[c].y : v[c].x;
+            b2[c] = imcol ? v[c].x : -v[c].y;
+        }
+        float gr[NM], gi[NM];
+#pragma unroll
+        for (int m = 0; m < NM; ++m) {
+            const float2 gv = green32(tx[m] - xs, ty[m] - ys, tz[m] - zs, KT);
+            gr[m] = gv.x;
+            gi[m] = gv.y;
+        }
+#pragma unroll
+        for (int m = 0; m < NM; ++m)
+#pragma unroll
+            for (int c = 0; c < NB; ++c) acc[m][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(gr[m], b1[c], acc[m][c], 0, 0, 0);
+#pragma unroll
+        for (int m = 0; m < NM; ++m)
+#pragma unroll
+            for (int c = 0; c < NB; ++c) acc[m][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(gi[m], b2[c], acc[m][c], 0, 0, 0);
+        j = jn;
+        xs = xn;
+        ys = yn;
+        zs = zn;
+#pragma unroll
+        for (int c = 0; c < NB; ++c) v[c] = vn[c];
+    }
+}
+
+template <int NB, int KT>
+__global__ __launch_bounds__(256) void kapply_mfma_kernel(
+    const KTask* __restrict__ tasks, const KSrc* __restrict__ srcs,
+    const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
+    BufTable bt, int task0, int ntask, int nrhs) {
+    constexpr int NR = 8 * NB;                       // right-hand sides of the block
+    __shared__ double sx[KM_ST], sy[KM_ST], sz[KM_ST];
+    // densities, right-hand side fastest, rows padded by one entry (the staging stores of
+    // consecutive sources then spread over the banks); after the last source tile, the
+    // partial sums of waves that split the sources (4 waves x 4 row tiles x 64 lanes x 4 NB)
+    __shared__ __align__(16) float2 sv[KM_ST][NR + 1];
+    static_assert(sizeof(float2) * KM_ST * (NR + 1) >= sizeof(float) * 4 * 4 * 64 * 4 * NB, "reduction fits the tile");
+    float* red = (float*)&sv[0][0];
+    const int r0 = blockIdx.y * NR;
+    const int nr = min(NR, nrhs - r0);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int g = lane >> 4, li = lane & 15;
+    for (int bi = blockIdx.x; bi < ntask; bi += gridDim.x) {
+    const KTask t = tasks[task0 + xcd_item(bi, ntask)];
+    for (int tb = 0; tb < t.count; tb += 256) {
+        const int nt = min(256, t.count - tb);
+        const int nT = (nt + 15) >> 4;
+        // up to 4 row tiles: every wave takes all of them and a quarter of the sources;
+        // up to 8: two waves per row tile set; beyond: the waves split the row tiles
+        const int wrt = nT > 8 ? 4 : nT > 4 ? 2 : 1, wsrc = 4 / wrt;
+        const int myrt0 = wave % wrt, mysrc = wave / wrt;
+        // this wave's row tiles: myrt0 + wrt * m < nT, m < nm
+        const int nm = min(4, (nT - myrt0 + wrt - 1) / wrt);
+        double tx[4], ty[4], tz[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const int i = (myrt0 + wrt * m) * 16 + li;
+            tx[m] = ty[m] = tz[m] = 0;
+            if (m < nm && i < nt) {
+                tx[m] = px[t.off + tb + i] * t.scale + t.c[0];
+                ty[m] = py[t.off + tb + i] * t.scale + t.c[1];
+                tz[m] = pz[t.off + tb + i] * t.scale + t.c[2];
+            }
+        }
+        kacc4 acc[4][NB];
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int c = 0; c < NB; ++c) acc[m][c] = kacc4{0, 0, 0, 0};
+        for (int si = t.s0; si < t.s1; ++si) {
+            const KSrc s = srcs[si];
+            const float2* vb = bufp<float2>(bt, s.vbuf) + s.voff;
+            const int64_t vs = bt.stride[s.vbuf];
+            for (int sb = 0; sb < s.count; sb += KM_ST) {
+                const int n = min(KM_ST, s.count - sb);
+                __syncthreads();
+                {   // sources past n: coordinates 0 (a finite G) and zero densities
+                    const int j = tid;
+                    double x = 0, y = 0, z = 0;
+                    if (j < n) {
+                        x = px[s.off + sb + j] * s.scale + s.c[0];
+                        y = py[s.off + sb + j] * s.scale + s.c[1];
+                        z = pz[s.off + sb + j] * s.scale + s.c[2];
+                    }
+                    sx[j] = x;
+                    sy[j] = y;
+                    sz[j] = z;
+#pragma unroll
+                    for (int q = 0; q < NR; ++q)
+                        sv[j][q] = (j < n && q < nr) ? vb[(int64_t)(r0 + q) * vs + sb + j] : make_float2(0.f, 0.f);
+                }
+                __syncthreads();
+                // groups of 4 sources: this wave takes groups mysrc, mysrc + wsrc, ...; the
+                // staged rows past n hold zero densities
+                const int ng = (n + 3) >> 2;
+                switch (nm) {      // wave-uniform
+                case 4: kmfma_steps<4, NB, KT>(acc, tx, ty, tz, sx, sy, sz, sv, mysrc, wsrc, ng, g, li); break;
+                case 3: kmfma_steps<3, NB, KT>(acc, tx, ty, tz, sx, sy, sz, sv, mysrc, wsrc, ng, g, li); break;
+                case 2: kmfma_steps<2, NB, KT>(acc, tx, ty, tz, sx, sy, sz, sv, mysrc, wsrc, ng, g, li); break;
+                case 1: kmfma_steps<1, NB, KT>(acc, tx, ty, tz, sx, sy, sz, sv, mysrc, wsrc, ng, g, li); break;
+                default: break;    // a wave with no row tile in this pass
+                }
+            }
+        }
+        // waves that split the sources of one row tile: sum their partials (fixed order)
+        if (wsrc > 1) {
+            __syncthreads();
+            auto slot = [&](int w, int m, int c, int v) { return ((w * 4 + m) * 64 + lane) * 4 * NB + 4 * c + v; };
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                if (m >= nm) break;
+#pragma unroll
+                for (int c = 0; c < NB; ++c)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) red[slot(wave, m, c, v)] = acc[m][c][v];
+            }
+            __syncthreads();
+            if (mysrc == 0) {
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    if (m >= nm) break;
+#pragma unroll
+                    for (int c = 0; c < NB; ++c)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            float a = red[slot(wave, m, c, v)];
+                            for (int h = 1; h < wsrc; ++h) a += red[slot(wave + h * wrt, m, c, v)];
+                            acc[m][c][v] = a;
+                        }
+                }
+            }
+        }
+        // D[row][col] of a 16x16 f32 MFMA: lane holds rows 4 * (lane / 16) + v, column lane % 16
+        if (mysrc == 0) {
+            float* ob = (float*)(bufp<float2>(bt, t.obuf) + t.ooff + tb);
+            const int64_t os = bt.stride[t.obuf];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                if (m >= nm) break;
+                const int rt = myrt0 + wrt * m;
+#pragma unroll
+                for (int c = 0; c < NB; ++c) {
+                    const int q = 8 * c + (li >> 1);
+                    if (q >= nr) continue;
+                    float* oq = ob + 2 * ((int64_t)(r0 + q) * os) + (li & 1);
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int i = rt * 16 + 4 * g + v;
+                        if (i >= nt) continue;
+                        if (t.mode == 0) oq[2 * i] = acc[m][c][v];
+                        else oq[2 * i] += acc[m][c][v];
+                    }
                 }
             }
         }
@@ -654,6 +865,11 @@ __device__ __forceinline__ void stage_pairs(const int* __restrict__ vpk, const V
     for (int e = t; e < n4; e += VL_THREADS) dst[e] = src[e];
 }
 
+// A block walks VL_SLICES consecutive frequency slices of one chunk: the chunk's packed
+// pairs are staged once, and the tensor and source slices of the next frequency slice
+// are loaded into registers while the current one is accumulated.
+constexpr int VL_GROUPS = 108;    // slice groups per chunk: 1728 / (F * slices per group)
+
 template <class V>
 __global__ __launch_bounds__(VL_THREADS) void vlist_acc_kernel(const VTarget* __restrict__ vt,
                                                                const VChunk* __restrict__ vc,
@@ -662,36 +878,40 @@ __global__ __launch_bounds__(VL_THREADS) void vlist_acc_kernel(const VTarget* __
                                                                const V* __restrict__ opm, BufTable bt, int v0,
                                                                int c0, int nchunk) {
     constexpr int G3 = 1728, F = VSlice<V>::F, SE = VSlice<V>::SE, TPP = VL_THREADS / F;
+    constexpr int NSL = G3 / F / VL_GROUPS;
+    constexpr int NT = (343 * F + VL_THREADS - 1) / VL_THREADS, NS = (VCH_SRC * F + VL_THREADS - 1) / VL_THREADS;
     __shared__ V tab[343 * SE], src[VCH_SRC * SE];
     __shared__ __align__(16) int pkl[VCH_PK];
     const int t = threadIdx.x, fl = t % F;
-    const int item = xcd_item(blockIdx.x, (G3 / F) * nchunk);
-    const int f0 = (item / nchunk) * F;
+    const int item = xcd_item(blockIdx.x, VL_GROUPS * nchunk);
+    const int fg = (item / nchunk) * NSL * F;
     const VChunk ch = vc[c0 + item % nchunk];
-    const V* T = opm + vt[v0].tab_off + f0;
-    const V* spec = bufp<V>(bt, BUF_SPEC) + f0;
-    {   // staging: every thread's loads are issued before any is used; row 0 of the
-        // source rows is zero
-        constexpr int NT = (343 * F + VL_THREADS - 1) / VL_THREADS, NS = (VCH_SRC * F + VL_THREADS - 1) / VL_THREADS;
-        const int nu = (ch.u1 - ch.u0) * F;
-        V a[NT], b[NS];
-        int si[NS];
+    const V* T = opm + vt[v0].tab_off;
+    const V* spec = bufp<V>(bt, BUF_SPEC);
+    const int nu = (ch.u1 - ch.u0) * F;
+    int64_t si[NS];                          // source row offsets, fixed for the block
 #pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            const int e = t + VL_THREADS * k;
-            si[k] = e < nu ? vsrc[ch.u0 + e / F] : 0;
-        }
+    for (int k = 0; k < NS; ++k) {
+        const int e = t + VL_THREADS * k;
+        si[k] = (e >= F && e < nu) ? (int64_t)vsrc[ch.u0 + e / F] * G3 + e % F : -1;
+    }
+    V a[NT], b[NS];
+    auto fetch = [&](int f0) {
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
             const int e = t + VL_THREADS * k;
-            a[k] = e < 343 * F ? T[(int64_t)(e / F) * G3 + e % F] : mk<V>(0, 0);
+            a[k] = e < 343 * F ? T[(int64_t)(e / F) * G3 + f0 + e % F] : mk<V>(0, 0);
         }
 #pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            const int e = t + VL_THREADS * k;
-            b[k] = e >= F && e < nu ? spec[(int64_t)si[k] * G3 + e % F] : mk<V>(0, 0);
-        }
-        stage_pairs(vpk, ch, pkl, t);
+        for (int k = 0; k < NS; ++k) b[k] = si[k] >= 0 ? spec[si[k] + f0] : mk<V>(0, 0);
+    };
+    fetch(fg);
+    stage_pairs(vpk, ch, pkl, t);
+    const V* srcl = src + fl;
+    const V* tabl = tab + fl;
+    for (int sl = 0; sl < NSL; ++sl) {
+        const int f0 = fg + sl * F;
+        __syncthreads();                     // the previous slice's reads of tab / src are done
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
             const int e = t + VL_THREADS * k;
@@ -700,56 +920,57 @@ __global__ __launch_bounds__(VL_THREADS) void vlist_acc_kernel(const VTarget* __
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
             const int e = t + VL_THREADS * k;
-            if (e < nu) src[(e / F) * SE + e % F] = b[k];
+            if (e < nu) src[(e / F) * SE + e % F] = b[k];     // row 0 stays zero (si = -1)
         }
-    }
-    __syncthreads();
-    const V* srcl = src + fl;
-    const V* tabl = tab + fl;
-    V* out = bufp<V>(bt, BUF_TSPEC) + f0 + fl;
-    for (int i = ch.t0 + t / F; i < ch.t1; i += TPP) {
-        const VTarget tg = vt[i];
-        V acc = mk<V>(0, 0);
-        // the target's packed pairs (zero pairs pad to a multiple of 8; row 0 is zero),
-        // read from LDS by every lane of its group (one address: a broadcast)
-        const int n = (tg.p1 - tg.p0 + 7) & ~7;
-        const int4* pk = (const int4*)(pkl + (tg.k0 - ch.k0));
-        for (int j = 0; j < n; j += F) {
-            V sv[F], tv[F];
+        __syncthreads();
+        if (sl + 1 < NSL) fetch(f0 + F);     // in flight during the pair loop below
+        V* out = bufp<V>(bt, BUF_TSPEC) + f0 + fl;
+        for (int i = ch.t0 + t / F; i < ch.t1; i += TPP) {
+            const VTarget tg = vt[i];
+            V acc = mk<V>(0, 0);
+            // the target's packed pairs (zero pairs pad to a multiple of 8; row 0 is zero),
+            // read from LDS by every lane of its group (one address: a broadcast)
+            const int n = (tg.p1 - tg.p0 + 7) & ~7;
+            const int4* pk = (const int4*)(pkl + (tg.k0 - ch.k0));
+            for (int j = 0; j < n; j += F) {
+                V sv[F], tv[F];
 #pragma unroll
-            for (int u = 0; u < F; ++u) {
-                const int4 q = pk[(j + u) / 4];
-                const int w = (u % 4 == 0) ? q.x : (u % 4 == 1) ? q.y : (u % 4 == 2) ? q.z : q.w;
-                sv[u] = srcl[(w & 0xffff) * SE];
-                tv[u] = tabl[(w >> 16) * SE];
+                for (int u = 0; u < F; ++u) {
+                    const int4 q = pk[(j + u) / 4];
+                    const int w = (u % 4 == 0) ? q.x : (u % 4 == 1) ? q.y : (u % 4 == 2) ? q.z : q.w;
+                    sv[u] = srcl[(w & 0xffff) * SE];
+                    tv[u] = tabl[(w >> 16) * SE];
+                }
+#pragma unroll
+                for (int u = 0; u < F; ++u) cfma(acc, sv[u], tv[u]);
             }
-#pragma unroll
-            for (int u = 0; u < F; ++u) cfma(acc, sv[u], tv[u]);
+            out[(int64_t)(i - v0) * G3] = acc;
         }
-        out[(int64_t)(i - v0) * G3] = acc;
     }
 }
 
 // V list, accumulation for several right-hand sides (nrhs > 1).  The source spectra are
 // stored right-hand side fastest, SPEC[(row * (2P)^3 + f) * nrhs + r], so the values of
 // one source for F frequencies and RC = R1 * R2 right-hand sides of a block are one
-// 128-byte run per frequency.  A block owns (chunk, slice of F frequencies, group of RC
-// right-hand sides) and stages the slice of the translation tensor (343 offsets, shared
-// by the group) and the union's source rows (128 bytes each).  Eight lanes work on one
-// target: lane l takes frequency l / R1 and the R2 right-hand sides (l % R1) * R2 + q,
-// so the eight lanes read one whole source row per pair (all 32 banks, conflict-free)
-// and one tensor value serves R2 complex FMAs.  The accumulated target spectra go to
-// TSPEC in the [rhs][target][(2P)^3] layout of the inverse transform.
+// 128-byte run per frequency.  A block owns (chunk, group of frequency slices of F
+// frequencies, group of RC right-hand sides) and stages the slice of the translation
+// tensor (343 offsets, shared by the group) and the union's source rows (128 bytes each).
+// Eight lanes work on one target: lane l takes frequency l / R1 and the R2 right-hand
+// sides (l % R1) * R2 + q, so the eight lanes read one whole source row per pair (all 32
+// banks) and one tensor value serves R2 complex FMAs.  The accumulated target spectra go
+// to TSPEC in the [rhs][target][(2P)^3] layout of the inverse transform.
 template <class V> struct VMulti { static constexpr int R2 = 16 / (int)sizeof(V); };   // float2: 2, double2: 1
 
 template <class V, int F, int R1>
 __global__ __launch_bounds__(VL_THREADS) void vlist_acc_multi_kernel(const VTarget* __restrict__ vt,
-                                                              const VChunk* __restrict__ vc,
-                                                              const int* __restrict__ vsrc,
-                                                              const int* __restrict__ vpk,
-                                                              const V* __restrict__ opm, BufTable bt, int v0,
-                                                              int c0, int nchunk, int nrhs) {
-    constexpr int G3 = 1728, R2 = VMulti<V>::R2, RC = R1 * R2, ROW = F * RC, NSL = G3 / F;
+                                                                     const VChunk* __restrict__ vc,
+                                                                     const int* __restrict__ vsrc,
+                                                                     const int* __restrict__ vpk,
+                                                                     const V* __restrict__ opm, BufTable bt, int v0,
+                                                                     int c0, int nchunk, int nrhs) {
+    constexpr int G3 = 1728, R2 = VMulti<V>::R2, RC = R1 * R2, ROW = F * RC;
+    constexpr int NSL = G3 / F / VL_GROUPS;
+    constexpr int NT = (343 * F + VL_THREADS - 1) / VL_THREADS, NS = (VCH_SRC * ROW + VL_THREADS - 1) / VL_THREADS;
     static_assert(F * R1 == 8, "eight lanes per target");
     static_assert(ROW * (int)sizeof(V) == 128, "128-byte source rows");
     __shared__ V tab[343 * F];
@@ -757,31 +978,39 @@ __global__ __launch_bounds__(VL_THREADS) void vlist_acc_multi_kernel(const VTarg
     __shared__ __align__(16) int pkl[VCH_PK];
     const int t = threadIdx.x, l = t & 7;
     const int fl = l / R1, rq = (l % R1) * R2;
-    const int item = xcd_item(blockIdx.x, NSL * nchunk);
-    const int f0 = (item / nchunk) * F;
+    const int item = xcd_item(blockIdx.x, VL_GROUPS * nchunk);
+    const int fg = (item / nchunk) * NSL * F;
     const int rg = blockIdx.y * RC;
     const VChunk ch = vc[c0 + item % nchunk];
-    const V* T0 = opm + vt[v0].tab_off + f0;
+    const V* T = opm + vt[v0].tab_off;
     const V* spec = bufp<V>(bt, BUF_SPEC);
-    {   // staging, all loads of a thread issued before its LDS writes; row 0 is zero
-        constexpr int NT = (343 * F + VL_THREADS - 1) / VL_THREADS, NS = (VCH_SRC * ROW + VL_THREADS - 1) / VL_THREADS;
-        const int nu = (ch.u1 - ch.u0) * ROW;
-        V a[NT], b[NS];
+    const int nu = (ch.u1 - ch.u0) * ROW;
+    int64_t si[NS];                          // element offsets at frequency 0, fixed for the block
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const int e = t + VL_THREADS * k;
+        const int u = e / ROW, w = e % ROW, f = w / RC, r = rg + w % RC;
+        si[k] = (e >= ROW && e < nu && r < nrhs) ? ((int64_t)vsrc[ch.u0 + u] * G3 + f) * nrhs + r : -1;
+    }
+    V a[NT], b[NS];
+    auto fetch = [&](int f0) {
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
             const int e = t + VL_THREADS * k;
-            a[k] = e < 343 * F ? T0[(int64_t)(e / F) * G3 + e % F] : mk<V>(0, 0);
+            a[k] = e < 343 * F ? T[(int64_t)(e / F) * G3 + f0 + e % F] : mk<V>(0, 0);
         }
 #pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            const int e = t + VL_THREADS * k;
-            const int u = e / ROW, w = e % ROW, f = w / RC, r = rg + w % RC;
-            V v = mk<V>(0, 0);
-            if (e >= ROW && e < nu && r < nrhs)
-                v = spec[((int64_t)vsrc[ch.u0 + u] * G3 + f0 + f) * nrhs + r];
-            b[k] = v;
-        }
-        stage_pairs(vpk, ch, pkl, t);
+        for (int k = 0; k < NS; ++k) b[k] = si[k] >= 0 ? spec[si[k] + (int64_t)f0 * nrhs] : mk<V>(0, 0);
+    };
+    fetch(fg);
+    stage_pairs(vpk, ch, pkl, t);
+    const V* tabl = tab + fl;
+    const V* srcl = src + l * R2;
+    V* out = bufp<V>(bt, BUF_TSPEC);
+    const int64_t os = bt.stride[BUF_TSPEC];
+    for (int sl = 0; sl < NSL; ++sl) {
+        const int f0 = fg + sl * F;
+        __syncthreads();
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
             const int e = t + VL_THREADS * k;
@@ -790,47 +1019,43 @@ __global__ __launch_bounds__(VL_THREADS) void vlist_acc_multi_kernel(const VTarg
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
             const int e = t + VL_THREADS * k;
-            if (e < nu) src[e] = b[k];
+            if (e < nu) src[e] = b[k];       // row 0 and right-hand sides past nrhs are zero
         }
-    }
-    __syncthreads();
-    const V* tabl = tab + fl;
-    const V* srcl = src + l * R2;
-    V* out = bufp<V>(bt, BUF_TSPEC);
-    const int64_t os = bt.stride[BUF_TSPEC];
-    for (int i = ch.t0 + t / 8; i < ch.t1; i += VL_THREADS / 8) {
-        const VTarget tg = vt[i];
-        V acc[R2];
+        __syncthreads();
+        if (sl + 1 < NSL) fetch(f0 + F);     // in flight during the pair loop below
+        for (int i = ch.t0 + t / 8; i < ch.t1; i += VL_THREADS / 8) {
+            const VTarget tg = vt[i];
+            V acc[R2];
 #pragma unroll
-        for (int q = 0; q < R2; ++q) acc[q] = mk<V>(0, 0);
-        const int n = (tg.p1 - tg.p0 + 7) & ~7;
-        const int4* pk = (const int4*)(pkl + (tg.k0 - ch.k0));
-        auto step = [&](const int4* q4) {
-            V tv[8], sv[8][R2];
+            for (int q = 0; q < R2; ++q) acc[q] = mk<V>(0, 0);
+            const int n = (tg.p1 - tg.p0 + 7) & ~7;
+            const int4* pk = (const int4*)(pkl + (tg.k0 - ch.k0));
+            for (int j = 0; j < n; j += 8) {
+                V tv[8], sv[8][R2];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int4 q = q4[u / 4];
-                const int w = (u % 4 == 0) ? q.x : (u % 4 == 1) ? q.y : (u % 4 == 2) ? q.z : q.w;
-                tv[u] = tabl[(w >> 16) * F];
-                const V* sp = srcl + (w & 0xffff) * ROW;
-                if constexpr (R2 == 2) {
-                    const float4 x = *(const float4*)sp;
-                    sv[u][0] = mk<V>(x.x, x.y);
-                    sv[u][R2 - 1] = mk<V>(x.z, x.w);
-                } else {
-                    sv[u][0] = sp[0];
+                for (int u = 0; u < 8; ++u) {
+                    const int4 q = pk[(j + u) / 4];
+                    const int w = (u % 4 == 0) ? q.x : (u % 4 == 1) ? q.y : (u % 4 == 2) ? q.z : q.w;
+                    tv[u] = tabl[(w >> 16) * F];
+                    const V* sp = srcl + (w & 0xffff) * ROW;
+                    if constexpr (R2 == 2) {
+                        const float4 x = *(const float4*)sp;
+                        sv[u][0] = mk<V>(x.x, x.y);
+                        sv[u][R2 - 1] = mk<V>(x.z, x.w);
+                    } else {
+                        sv[u][0] = sp[0];
+                    }
                 }
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+#pragma unroll
+                    for (int q = 0; q < R2; ++q) cfma(acc[q], sv[u][q], tv[u]);   // operand order of the 1-RHS kernel
             }
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
-#pragma unroll
-                for (int q = 0; q < R2; ++q) cfma(acc[q], sv[u][q], tv[u]);   // operand order of the 1-RHS kernel
-        };
-        for (int j = 0; j < n; j += 8) step(pk + j / 4);
-#pragma unroll
-        for (int q = 0; q < R2; ++q) {
-            const int r = rg + rq + q;
-            if (r < nrhs) out[(int64_t)r * os + (int64_t)(i - v0) * G3 + f0 + fl] = acc[q];
+            for (int q = 0; q < R2; ++q) {
+                const int r = rg + rq + q;
+                if (r < nrhs) out[(int64_t)r * os + (int64_t)(i - v0) * G3 + f0 + fl] = acc[q];
+            }
         }
     }
 }
@@ -1082,6 +1307,22 @@ hipError_t launch_stage_t(const DevPlan& d, const Stage& s, int nrhs, hipStream_
         break;
     case ST_KAPPLY: {
         const int gx = s.stream ? std::min(n, AUX_KAPPLY_BLOCKS) : n;
+        if constexpr (sizeof(V) == 8) if (d.kmfma) {   // mixed precision: the products on the matrix cores
+            const int nb = nrhs > 8 ? 2 : 1;
+            const dim3 grid(gx, (nrhs + 8 * nb - 1) / (8 * nb));
+            if (nb == 1) {
+                if (d.ktype == 0)
+                    kapply_mfma_kernel<1, 0><<<grid, 256, 0, st>>>(d.ktasks, d.ksrcs, d.px, d.py, d.pz, d.bt, s.begin, n, nrhs);
+                else
+                    kapply_mfma_kernel<1, 1><<<grid, 256, 0, st>>>(d.ktasks, d.ksrcs, d.px, d.py, d.pz, d.bt, s.begin, n, nrhs);
+            } else {
+                if (d.ktype == 0)
+                    kapply_mfma_kernel<2, 0><<<grid, 256, 0, st>>>(d.ktasks, d.ksrcs, d.px, d.py, d.pz, d.bt, s.begin, n, nrhs);
+                else
+                    kapply_mfma_kernel<2, 1><<<grid, 256, 0, st>>>(d.ktasks, d.ksrcs, d.px, d.py, d.pz, d.bt, s.begin, n, nrhs);
+            }
+            break;
+        }
         if (nrhs == 1) {
             kapply_kernel<V, 1><<<dim3(gx, 1), KA_THREADS, 0, st>>>(d.ktasks, d.ksrcs, d.px, d.py, d.pz, d.bt,
                                                                     s.begin, n, nrhs, d.ktype);
@@ -1120,20 +1361,19 @@ hipError_t launch_stage_t(const DevPlan& d, const Stage& s, int nrhs, hipStream_
         const int nchunk = (int)s.count;
         const V* opm = d.mixed ? (const V*)d.opmat32 : (const V*)d.opmat;
         if (nrhs == 1) {
-            constexpr int F = VSlice<V>::F;
-            vlist_acc_kernel<V><<<dim3((1728 / F) * nchunk, 1), VL_THREADS, 0, st>>>(d.vtrg, d.vchunk, d.vsrc, d.vpk, opm,
+            vlist_acc_kernel<V><<<dim3(VL_GROUPS * nchunk, 1), VL_THREADS, 0, st>>>(d.vtrg, d.vchunk, d.vsrc, d.vpk, opm,
                                                                               d.bt, s.begin, s.buf, nchunk);
         } else {
             // right-hand sides per block: 16 / 8 / 4 (FP32) or 8 / 4 / 2 (FP64) by nrhs
             constexpr int R2 = VMulti<V>::R2;
             if (nrhs >= 8 * R2) {
-                vlist_acc_multi_kernel<V, 1, 8><<<dim3(1728 * nchunk, (nrhs + 8 * R2 - 1) / (8 * R2)), VL_THREADS, 0, st>>>(
+                vlist_acc_multi_kernel<V, 1, 8><<<dim3(VL_GROUPS * nchunk, (nrhs + 8 * R2 - 1) / (8 * R2)), VL_THREADS, 0, st>>>(
                     d.vtrg, d.vchunk, d.vsrc, d.vpk, opm, d.bt, s.begin, s.buf, nchunk, nrhs);
             } else if (nrhs >= 4 * R2) {
-                vlist_acc_multi_kernel<V, 2, 4><<<dim3(864 * nchunk, (nrhs + 4 * R2 - 1) / (4 * R2)), VL_THREADS, 0, st>>>(
+                vlist_acc_multi_kernel<V, 2, 4><<<dim3(VL_GROUPS * nchunk, (nrhs + 4 * R2 - 1) / (4 * R2)), VL_THREADS, 0, st>>>(
                     d.vtrg, d.vchunk, d.vsrc, d.vpk, opm, d.bt, s.begin, s.buf, nchunk, nrhs);
             } else {
-                vlist_acc_multi_kernel<V, 4, 2><<<dim3(432 * nchunk, (nrhs + 2 * R2 - 1) / (2 * R2)), VL_THREADS, 0, st>>>(
+                vlist_acc_multi_kernel<V, 4, 2><<<dim3(VL_GROUPS * nchunk, (nrhs + 2 * R2 - 1) / (2 * R2)), VL_THREADS, 0, st>>>(
                     d.vtrg, d.vchunk, d.vsrc, d.vpk, opm, d.bt, s.begin, s.buf, nchunk, nrhs);
             }
         }

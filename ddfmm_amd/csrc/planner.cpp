@@ -649,9 +649,15 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
     pl.stats.nlevels_hf = (int)hfl.size();
     double maxW = hfl.empty() ? 0 : hfl.rbegin()->first;
     pl.stats.max_hf_width = (int)maxW;
+    // Out-slots (directional upward equivalents) of every level stay live from the upward
+    // pass to the M2L of their level.  In-slots (downward check values) of width W are
+    // written by the level's reduction and last read by its L2L, which precedes the next
+    // finer level's reduction on the same stream, so every level's in-slots start at 0:
+    // BUF_HF_DN holds one level (the level-by-level schedule of the reference's
+    // LIMITED_MEMORY build, wave3d_eval.cpp:106-186, applied to the downward data).
     for (auto& lv : hfl) {
         double W = lv.first;
-        int64_t lvl_o = 0, lvl_i = 0;
+        int64_t lvl_o = 0, lvl_i = 0, lvl_dn = 0;
         for (auto& h : lv.second) {
             int rm = std::max(h.r1, h.r2);
             for (auto& k : h.out) {
@@ -659,9 +665,10 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
                 oscr[{k, h.dir}] = lvl_o; lvl_o += rm;
             }
             for (auto& k : h.in) {
-                inslot[{k, h.dir}] = hf_dn_size; hf_dn_size += h.r1;
+                inslot[{k, h.dir}] = lvl_dn; lvl_dn += h.r1;
                 iscr[{k, h.dir}] = lvl_i; lvl_i += rm;
             }
+            hf_dn_size = std::max(hf_dn_size, lvl_dn);
             pl.stats.nhf_out += (int64_t)h.out.size();
             pl.stats.nhf_in += (int64_t)h.in.size();
             // operators: E1, E2, E3 uploaded once per canonical entry

@@ -123,6 +123,7 @@ class Wave3d:
         self.maxlevel = None
         self.nrhs_max = 1
         self.precision = FP64
+        self.hbm_budget = 0.0          # GiB; 0 = no budget (ddfmm_set_hbm_budget)
         self._ctx = None
 
     def setup(self, opts: dict | None = None, nrhs_max: int | None = None,
@@ -137,6 +138,7 @@ class Wave3d:
             self.ctr = tuple(float(x) for x in str(opts[p + "ctr"]).split())
         self.ptsmax = _opt(opts, p + "ptsmax", int, self.ptsmax)
         self.maxlevel = _opt(opts, p + "maxlevel", int, self.maxlevel)
+        self.hbm_budget = _opt(opts, p + "hbm_budget", float, self.hbm_budget)
         if nrhs_max is not None:
             self.nrhs_max = int(nrhs_max)
         if precision is not None:
@@ -157,6 +159,7 @@ class Wave3d:
         check(lib().ddfmm_create(self.device, ctypes.byref(h)))
         self._ctx = h
         self._n = pos.shape[0]
+        check(lib().ddfmm_set_hbm_budget(h, int(self.hbm_budget * 2**30)))
         check(lib().ddfmm_setup_mlib(h, ptr(pos), pos.shape[0], ctypes.byref(prm), self.mlibptr.handle))
         return 0
 
@@ -227,6 +230,14 @@ class Wave3d:
     def set_tiles(self, table) -> None:
         t = (ctypes.c_int * max(len(table), 1))(*table)
         check(lib().ddfmm_set_tiles(self._ctx, len(table), t))
+
+    def memory(self) -> dict:
+        """Device bytes of the set-up context (ddfmm_memory): tables, one right-hand side's
+        work buffers and staging, setup-only generator tables, and the right-hand sides
+        per pass under the HBM budget."""
+        f, r, t, s = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
+        check(lib().ddfmm_memory(self._ctx, ctypes.byref(f), ctypes.byref(r), ctypes.byref(t), ctypes.byref(s)))
+        return {"fixed_bytes": f.value, "per_rhs_bytes": r.value, "setup_bytes": t.value, "slab": s.value}
 
     def set_graph(self, on: bool = True):
         """Capture the evaluation of each right-hand-side count once into a HIP graph and

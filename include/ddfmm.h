@@ -122,6 +122,21 @@ DDFMM_API int ddfmm_autotune(ddfmm_ctx* ctx, int nrhs, int reps, int cap, int* m
 DDFMM_API int ddfmm_set_tiles(ddfmm_ctx* ctx, int count, const int* mbmax);
 DDFMM_API int ddfmm_get_tiles(ddfmm_ctx* ctx, int cap, int* mbmax, char* names /* cap x 24 */, int* count);
 
+/* HBM budget (the reference's LIMITED_MEMORY build, wave3d_eval.cpp:27-48 and 106-186,
+   bounds its directional data level by level).  The downward directional check values
+   always occupy one level at a time.  With a budget (bytes, applied at the next setup;
+   0 = none), setup sizes the work buffers for the largest right-hand-side count `slab`
+   (<= nrhs_max) that fits beside the tables, and ddfmm_eval / ddfmm_eval_device with
+   more right-hand sides run in passes of `slab`; results are the same bits as in one
+   pass.  Setup fails with DDFMM_ENOMEM, naming the minimum, when not even one
+   right-hand side fits.  ddfmm_memory reports the device bytes of the tables (fixed),
+   of one right-hand side's work buffers and staging (per_rhs), of the setup-only
+   generator tables (setup) and the slab; also on host-only contexts, where it is the
+   plan's requirement. */
+DDFMM_API int ddfmm_set_hbm_budget(ddfmm_ctx* ctx, int64_t bytes);
+DDFMM_API int ddfmm_memory(ddfmm_ctx* ctx, int64_t* fixed_bytes, int64_t* per_rhs_bytes, int64_t* setup_bytes,
+                           int* slab);
+
 /* Direct FP64 sum at targets (Wave3d::check's oracle sum, wave3d_check.cpp:66-76):
    out[r][i] = sum_j G(trg_i, src_j) den[r][j]; host buffers; points as rows. */
 DDFMM_API int ddfmm_direct_sum(int kernel, const double* trg, int m, const double* src, int n,

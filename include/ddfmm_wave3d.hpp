@@ -17,7 +17,9 @@
 // One process drives one GPU (the device of -wave3d_device, default 0); the cell
 // partition geomprtn only fixes the cell level (its edge length), as all cells are this
 // rank's.  Beyond the reference: -wave3d_precision (0 FP64, 1 mixed, 2 FP64 with fused
-// solves, default 0), eval_multi for several density vectors in one evaluation, and
+// solves, default 0), -wave3d_hbm_budget (GiB of device memory the context may hold; the
+// LIMITED_MEMORY build's memory bound, see ddfmm_set_hbm_budget), eval_multi for several
+// density vectors in one evaluation, and
 // handle() for the C ABI.  Every eval is a fresh evaluation (the reference's is
 // single-shot, wave3d_eval.cpp:518-521).  Errors throw a heap-allocated ddfmm::Error
 // (the reference throws `new std::exception()`, commoninc.hpp:52-53).
@@ -158,9 +160,10 @@ public:
     int& nrhs_max() { return _nrhs_max; }
     int& precision() { return _precision; }
     int& device() { return _device; }
+    double& hbm_budget() { return _hbm_budget; }
 
     // Wave3d::setup (wave3d_setup.cpp:21-97); option keys -<prefix>ACCU/NPQ/K/ctr/ptsmax/
-    // maxlevel (+ precision, nrhs_max, device)
+    // maxlevel (+ precision, nrhs_max, device, hbm_budget)
     int setup(std::map<std::string, std::string>& opts) {
         auto get = [&](const char* k) { return find_opt(opts, "-" + _prefix + k); };
         std::string v;
@@ -176,6 +179,7 @@ public:
         if (!(v = get("precision")).empty()) _precision = std::stoi(v);
         if (!(v = get("nrhs_max")).empty()) _nrhs_max = std::stoi(v);
         if (!(v = get("device")).empty()) _device = std::stoi(v);
+        if (!(v = get("hbm_budget")).empty()) _hbm_budget = std::stod(v);
         if (!_posptr || !_mlibptr) raise("Wave3d::setup: posptr and mlibptr must be set");
         if (_maxlevel <= 0) raise("Wave3d::setup: maxlevel must be given");
         if (!_mlibptr->handle()) raise("Wave3d::setup: Mlib3d::setup has not run");
@@ -201,6 +205,7 @@ public:
             _ctx = nullptr;
         }
         check_rc(ddfmm_create(_device, &_ctx));
+        check_rc(ddfmm_set_hbm_budget(_ctx, (int64_t)(_hbm_budget * 1073741824.0)));
         check_rc(ddfmm_setup_mlib(_ctx, xyz.data(), (int64_t)_keys.size(), &p, _mlibptr->handle()));
         return 0;
     }
@@ -267,6 +272,7 @@ private:
     // defaults of Wave3d::Wave3d (wave3d.cpp:24-26); maxlevel has none there
     int _ACCU = 1, _NPQ = 4, _ptsmax = 100, _maxlevel = 0, _nrhs_max = 1, _precision = DDFMM_FP64;
     double _K = 64;
+    double _hbm_budget = 0;
     Point3 _ctr{0.0, 0.0, 0.0};
     std::vector<int> _keys;
 };

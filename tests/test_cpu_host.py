@@ -212,3 +212,37 @@ def test_retile_keeps_useful_work(oplib):
             _lib.check(L.ddfmm_set_tiles(h, nl - 1, bad))
     finally:
         L.ddfmm_destroy(h)
+
+
+def test_hbm_budget_sets_the_slab(oplib):
+    """Under an HBM budget the context keeps the tables and as many right-hand sides'
+    work buffers as fit (the slab); below tables + one right-hand side, setup fails and
+    names the minimum."""
+    from ddfmm_amd.wave3d import Mlib3d, Wave3d, MIXED
+
+    pos = geometry.sphere(20000, 8.0, 3)
+    ml = Mlib3d()
+    ml.setup({"-mlib3d_ldname": oplib[0], "-mlib3d_hdname": oplib[1]})
+
+    def make(budget_gib):
+        w = Wave3d(device=-1)
+        w.posptr, w.mlibptr = pos, ml
+        w.setup({"-wave3d_ACCU": 2, "-wave3d_K": 8.0, "-wave3d_maxlevel": 12,
+                 "-wave3d_hbm_budget": budget_gib}, nrhs_max=5, precision=MIXED)
+        return w
+
+    w = make(0)
+    m = w.memory()
+    st = w.stats()
+    w.close()
+    assert m["slab"] == 5 and m["fixed_bytes"] > 0 and m["per_rhs_bytes"] > 0
+    # every leaf-sorted point: densities, potentials and their staging at least
+    assert m["per_rhs_bytes"] >= 4 * 20000 * 8
+    G = 1 << 30
+    for k, want in ((2.5, 2), (4.999, 4), (7.0, 5)):
+        w = make((m["fixed_bytes"] + k * m["per_rhs_bytes"]) / G)
+        assert w.memory()["slab"] == want, (k, w.memory())
+        w.close()
+    with pytest.raises(RuntimeError, match="HBM budget"):
+        make((m["fixed_bytes"] + 0.5 * m["per_rhs_bytes"]) / G)
+    assert st["nhf_in"] > 0

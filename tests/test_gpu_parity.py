@@ -120,6 +120,29 @@ def test_multi_rhs_eleven_mixed(oplib):
     w.close()
 
 
+@pytest.mark.parametrize("precision", [MIXED, FP64])
+@pytest.mark.parametrize("ptsmax", [100, 400])
+def test_rhs_counts_keep_bits(oplib, precision, ptsmax):
+    """Every kernel instance chosen by the right-hand-side count (kernel apply with 8 or 16
+    RHS per block, V list with 1, 2, 4 or 8 RHS per lane group, FFT with 1 or a block of
+    RHS) gives each right-hand side the same bits: 1, 2, 8, 9 and 16 RHS, leaves of up to
+    100 or 400 points (kernel-apply tasks of 1-16 row tiles)."""
+    K, N = 8.0, 30000
+    pos = geometry.sphere(N, K, 7)
+    den = np.atleast_2d(geometry.densities(N, 8, nrhs=16))
+    ml = Mlib3d()
+    ml.setup({"-mlib3d_ldname": oplib[0], "-mlib3d_hdname": oplib[1]})
+    w = Wave3d()
+    w.posptr, w.mlibptr = pos, ml
+    w.setup({"-wave3d_ACCU": 2, "-wave3d_K": K, "-wave3d_ptsmax": ptsmax, "-wave3d_maxlevel": 12}, nrhs_max=16,
+            precision=precision)
+    all16 = w.eval(den)
+    assert np.array_equal(w.eval(den), all16)
+    for n in (1, 2, 8, 9):
+        assert np.array_equal(w.eval(den[:n]), all16[:n]), n
+    w.close()
+
+
 def test_repeated_eval_is_fresh(oplib):
     K, N = 4.0, 4000
     pos = geometry.sphere(N, K, 3)

@@ -1,6 +1,9 @@
 """Stage-by-stage GPU parity: intermediate device buffers vs the oracle's intermediates
-(LF upward equivalent densities, HF directional equivalent densities and check values,
-LF downward equivalent densities, leaf potentials).
+(LF upward equivalent densities, HF directional equivalent densities, the width-1 HF
+check values, LF downward equivalent densities, leaf potentials).  The HF check values
+of a level occupy one level-sized region that the next finer level reuses, so after an
+evaluation the region holds width 1's, which depend on every coarser level through the
+L2L chain.
 
 Equivalent densities come out of ill-conditioned solves (cond G(ucp, uep) ~ 3e8 for the
 LF surfaces, cond R1 ~ 1e6 for the HF chain), so two FP64 evaluations that differ only in
@@ -32,15 +35,17 @@ def slot_maps(ora, hd, NL):
     for d in sorted(hdmap):
         byw.setdefault(max(abs(v) for v in d) // 4, []).append(d)
     up, dn = {}, {}
-    cu = cd = 0
+    cu = 0
     for W in sorted(byw):
+        cd = 0                      # in-slots are level-local
         for d in byw[W]:
             r1 = hd[float(W)][R.sort_dir(d)[0]]["uep"].shape[1]
             for k in hdmap[d][0]:
                 up[(k, d)] = (cu, r1)
                 cu += r1
             for k in hdmap[d][1]:
-                dn[(k, d)] = (cd, r1)
+                if W == 1:
+                    dn[(k, d)] = (cd, r1)
                 cd += r1
     return lfslot, up, dn
 

@@ -42,26 +42,35 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(ops_paths, kernel):
-    """The oracle (numpy FP64 FMM) on the 60k-point sphere of diameter 8 (K = 8)."""
-    from threadpoolctl import threadpool_limits
-
+def cpu_baseline(ops_paths, kernel, workload="sphere32_1m"):
+    """The compiled FP64 restatement of the FMM (oracle/wave3d_ref.cpp, OpenMP) on the host
+    cores: one right-hand side of `workload` (the reference's eval is single-RHS), setup
+    excluded, eval timed; threads = the cores this process may use, at most 16."""
     from ddfmm_amd import geometry, serialize
-    from oracle import wave3d_ref
+    from oracle import wave3d_cref, wave3d_ref
 
     cores = min(16, len(os.sched_getaffinity(0)))
-    N, K = 60_000, 8.0
-    pos = geometry.sphere(N, K, 0)
-    den = geometry.densities(N, 0)
+    wl = geometry.WORKLOADS[workload]
+    pos = geometry.workload_points(workload)
+    den = geometry.densities(len(pos), wl["seed"], nrhs=1)
     ld, hd = serialize.read_ld(ops_paths[0]), serialize.read_hd(ops_paths[1])
-    with threadpool_limits(limits=cores):
-        w = wave3d_ref.Wave3dOracle(pos, K, ld, hd, ktype=kernel).setup()
-        t0 = time.perf_counter()
-        w.eval(den)
-        dt = time.perf_counter() - t0
-    return {"value": N / dt / 1e6, "unit": "M point-evals/s·RHS", "cores": cores, "kind": "port",
-            "sample": f"oracle numpy FMM eval (FP64) on the sphere d=8 N=60000 K=8 workload, "
-                      f"one RHS, {dt:.1f} s"}
+    t0 = time.perf_counter()
+    c = wave3d_cref.Wave3dCOracle(pos, wl["K"], ld, hd, ktype=kernel, ptsmax=wl["ptsmax"], maxlevel=14,
+                                  numC=wl.get("numC", 1)).setup()
+    setup_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    val = c.eval(den, threads=cores)
+    dt = time.perf_counter() - t0
+    c.close()
+    chk = geometry.check_targets(len(pos), 200, wl["seed"])
+    err = wave3d_ref.relerr(val[chk], wave3d_ref.direct_sum(pos[chk], pos, den, kernel, threads=cores))
+    log(f"cpu baseline: {workload} 1 RHS on {cores} threads: eval {dt:.1f}s (setup {setup_s:.1f}s), "
+        f"rel.err vs direct {err:.2e}")
+    return {"value": len(pos) / dt / 1e6, "unit": "M point-evals/s·RHS", "cores": cores, "kind": "port",
+            "sample": f"compiled FP64 OpenMP restatement of Wave3d::setup+eval (oracle/wave3d_ref.cpp) on "
+                      f"{workload} ({len(pos)} points, K={wl['K']:g}), one RHS, eval only: {dt:.1f} s "
+                      f"(setup {setup_s:.1f} s), rel.err {err:.1e} vs FP64 direct at 200 targets",
+            "workload": workload, "eval_s": dt}
 
 
 def serve_bench(names, qps, count, slot, precision, seed=4):
@@ -109,6 +118,7 @@ KNAME = {
     "gemm_solve": lambda vt, rc: "gemm_kernel<double2,float2,double>" if vt == "float2"
     else "gemm_kernel<double2,double2,double>",
     "kapply": lambda vt, rc: f"kapply_kernel<{vt},{rc}>",
+    "kapply_mfma": lambda nb, kt: f"kapply_mfma_kernel<{nb},{kt}>",
     "vlist": lambda vt, rc: f"vlist_acc_kernel<{vt},{rc}>",
     "fft_fwd": lambda vt, rc: f"fft_fwd_kernel<{vt}>",
     "reduce": lambda vt, rc: f"reduce_kernel<{vt}>",
@@ -192,14 +202,14 @@ def measure(workload, precision, kernel, nrhs, steps, warmup, nchk, detail=False
            "n_check": int(len(chk)), "setup_s": setup_s, "device_GiB": st["device_bytes"] / 2**30,
            "autotune": tuned, "graph": bool(graph)}
     if detail:
-        out.update(stage_detail(w, st, dd, vd, nrhs, precision, workload))
+        out.update(stage_detail(w, st, dd, vd, nrhs, precision, workload, kernel))
     hip.free(dd)
     hip.free(vd)
     w.close()
     return out
 
 
-def stage_detail(w, st, dd, vd, nrhs, precision, workload):
+def stage_detail(w, st, dd, vd, nrhs, precision, workload, kernel=0):
     """Per-stage HIP-event times on the stages' own streams, the kernel classes and the
     roofline of the dominant class (work per launch / average launch duration)."""
     w.set_timing(True)
@@ -233,6 +243,9 @@ def stage_detail(w, st, dd, vd, nrhs, precision, workload):
     esz = 8 if mixed else 16
     vt = "float2" if mixed else "double2"
     kname = KNAME.get(dom, lambda vt, rc: dom)(vt, rhs_block(nrhs))
+    kmfma = mixed and nrhs >= 4         # contexts of >= 4 RHS apply kernels on the matrix cores
+    if dom == "kapply" and kmfma:
+        kname = KNAME["kapply_mfma"](2 if nrhs > 8 else 1, kernel)
     if dom in ("gemm_translation", "gemm_solve"):
         peak = PEAK["fp32"] if (mixed and dom == "gemm_translation") else PEAK["fp64"]
         roof = {"bound": "mfma", "achieved": per[dom]["work"] / t_dom / 1e12, "peak": peak / 1e12,
@@ -241,8 +254,11 @@ def stage_detail(w, st, dd, vd, nrhs, precision, workload):
     elif dom == "kapply":
         work = per[dom]["work"] / nrhs * (20 + 8 * nrhs)
         peak = PEAK["fp32"] if mixed else PEAK["fp64"]
-        roof = {"bound": "valu", "achieved": work / t_dom / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s"}
-        work_desc = "target-source interactions x (20 + 8*nrhs) flops on the vector ALU"
+        roof = {"bound": "mfma" if kmfma else "valu", "achieved": work / t_dom / 1e12, "peak": peak / 1e12,
+                "unit": "TFLOP/s"}
+        work_desc = ("target-source interactions x (20 + 8*nrhs) flops: G on the vector ALU, the "
+                     "8*nrhs products on the matrix cores (FP32)" if kmfma else
+                     "target-source interactions x (20 + 8*nrhs) flops on the vector ALU")
     elif dom == "reduce":
         work = per[dom]["work"] * esz
         roof = {"bound": "hbm", "achieved": work / t_dom / 1e9, "peak": PEAK["hbm"] / 1e9, "unit": "GB/s"}
@@ -302,6 +318,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the planner's GEMM tiling")
     ap.add_argument("--no-graph", action="store_true", help="issue the stages one by one (no graph replay)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-workload", default="sphere32_1m",
+                    help="workload of the CPU baseline (one RHS; the headline's 4M points take minutes)")
     ap.add_argument("--serve", default="sphere8_60k,sphere16_250k_x64,sphere32_1m,ellipsoid_4m_x8",
                     help="comma-separated geometries for the serving leg ('' to skip)")
     ap.add_argument("--serve-qps", type=float, default=10.0)
@@ -344,7 +362,11 @@ def main():
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
         cpu = cpu_baseline(mlib_gen.library_paths(os.path.join(ROOT, "ddfmm_amd", "_ops"), args.kernel, 2, 4),
-                           args.kernel)
+                           args.kernel, args.cpu_workload)
+        same = head if args.cpu_workload == args.workload else \
+            {"value": extra.get(args.cpu_workload, {}).get("mixed_value")}
+        if same.get("value"):
+            cpu["gpu_mixed_value_same_workload"] = same["value"]
         log(f"cpu baseline {cpu['value']:.3f} M/s on {cpu['cores']} cores")
 
     err = head["rel_err_vs_direct"]

@@ -27,7 +27,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "hbm": 8.0e12}
+PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15, "hbm": 8.0e12}
+PRECISIONS = {"fp64": 0, "mixed": 1, "fp64_fused": 2, "mixed_split": 3}   # ddfmm.h precision modes
 ST_ZERO, ST_GATHER, ST_SCATTER, ST_KAPPLY, ST_GEMM, ST_FFTFWD, ST_VLIST, ST_REDUCE, ST_FORK, ST_JOIN = range(10)
 
 
@@ -42,7 +43,7 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(ops_paths, kernel, workload="sphere32_1m"):
+def cpu_baseline(ops_paths, kernel, workload="ellipsoid_4m_x8"):
     """The compiled FP64 restatement of the FMM (oracle/wave3d_ref.cpp, OpenMP) on the host
     cores: one right-hand side of `workload` (the reference's eval is single-RHS), setup
     excluded, eval timed; threads = the cores this process may use, at most 16."""
@@ -77,7 +78,6 @@ def serve_bench(names, qps, count, slot, precision, seed=4):
     """Poisson request stream over resident geometries (continuous batching)."""
     from ddfmm_amd import geometry, mlib_gen
     from ddfmm_amd.serve import GpuEngine, Server, latency_stats, poisson_requests
-    from ddfmm_amd.wave3d import FP64, FP64_FUSED, MIXED
 
     ops = mlib_gen.ensure_library(os.path.join(ROOT, "ddfmm_amd", "_ops"), ktype=0, accu=2, NPQ=4,
                                   hf_max_W=4, lf_min_W=2.0 ** -3)
@@ -87,7 +87,7 @@ def serve_bench(names, qps, count, slot, precision, seed=4):
         wl = geometry.WORKLOADS[key]
         return GpuEngine(key, pts[key], {"-wave3d_ACCU": 2, "-wave3d_K": wl["K"], "-wave3d_ptsmax": wl["ptsmax"],
                                          "-wave3d_maxlevel": 14}, ops, slot,
-                         {"mixed": MIXED, "fp64": FP64, "fp64_fused": FP64_FUSED}[precision],
+                         PRECISIONS[precision],
                          numC=wl.get("numC", 1))
 
     srv = Server(make, lambda key: 0, hbm_budget=200 << 30)
@@ -138,7 +138,7 @@ def measure(workload, precision, kernel, nrhs, steps, warmup, nchk, detail=False
     roofline."""
     from ddfmm_amd import geometry, mlib_gen
     from ddfmm_amd._lib import Hip
-    from ddfmm_amd.wave3d import FP64, FP64_FUSED, MIXED, Kernel3d, Mlib3d, Wave3d, direct_sum
+    from ddfmm_amd.wave3d import Kernel3d, Mlib3d, Wave3d, direct_sum
 
     wl = dict(geometry.WORKLOADS[workload])
     nrhs = nrhs or wl["nrhs"]
@@ -157,7 +157,7 @@ def measure(workload, precision, kernel, nrhs, steps, warmup, nchk, detail=False
     t0 = time.perf_counter()
     w.setup({"-wave3d_ACCU": 2, "-wave3d_NPQ": 4, "-wave3d_K": wl["K"], "-wave3d_ptsmax": wl["ptsmax"],
              "-wave3d_maxlevel": 14}, nrhs_max=nrhs,
-            precision={"mixed": MIXED, "fp64": FP64, "fp64_fused": FP64_FUSED}[precision])
+            precision=PRECISIONS[precision])
     setup_s = time.perf_counter() - t0
     st = w.stats()
     log(f"setup {setup_s:.1f}s: leaves={st['nleaf']} m2l_pairs={st['m2l_pairs']} "
@@ -239,7 +239,8 @@ def stage_detail(w, st, dd, vd, nrhs, precision, workload, kernel=0):
     dom = max(per, key=lambda k: per[k]["ms"])
     t_dom = per[dom]["ms"] * 1e-3
     launches = per[dom]["launches"]
-    mixed = precision == "mixed"
+    mixed = precision in ("mixed", "mixed_split")
+    split = precision == "mixed_split"
     esz = 8 if mixed else 16
     vt = "float2" if mixed else "double2"
     kname = KNAME.get(dom, lambda vt, rc: dom)(vt, rhs_block(nrhs))
@@ -248,9 +249,14 @@ def stage_detail(w, st, dd, vd, nrhs, precision, workload, kernel=0):
         kname = KNAME["kapply_mfma"](2 if nrhs > 8 else 1, kernel)
     if dom in ("gemm_translation", "gemm_solve"):
         peak = PEAK["fp32"] if (mixed and dom == "gemm_translation") else PEAK["fp64"]
+        work_desc = "useful complex-GEMM flops 8*M*K*N over the class's groups"
+        if split and dom == "gemm_translation":
+            # six bf16 products per FP32 product: the useful-flop ceiling is the bf16 peak / 6
+            peak = PEAK["bf16"] / 6
+            kname = "gemm_bf16s_kernel"
+            work_desc += " (split-BF16: 6 bf16 MFMA products each; peak = dense bf16 peak / 6)"
         roof = {"bound": "mfma", "achieved": per[dom]["work"] / t_dom / 1e12, "peak": peak / 1e12,
                 "unit": "TFLOP/s"}
-        work_desc = "useful complex-GEMM flops 8*M*K*N over the class's groups"
     elif dom == "kapply":
         work = per[dom]["work"] / nrhs * (20 + 8 * nrhs)
         peak = PEAK["fp32"] if mixed else PEAK["fp64"]
@@ -307,7 +313,7 @@ def main():
                     help="headline workload (default: the largest single-GPU configuration)")
     ap.add_argument("--kernel", type=int, default=0)
     ap.add_argument("--nrhs", type=int, default=None)
-    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed", "fp64_fused"])
+    ap.add_argument("--precision", default="mixed", choices=sorted(PRECISIONS))
     ap.add_argument("--no-fp64-leg", action="store_true",
                     help="skip the plain-FP64 run of the headline workload (the mixed/FP64 ratio)")
     ap.add_argument("--extra", default="sphere32_1m,sphere16_250k_x64",
@@ -318,8 +324,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the planner's GEMM tiling")
     ap.add_argument("--no-graph", action="store_true", help="issue the stages one by one (no graph replay)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-workload", default="sphere32_1m",
-                    help="workload of the CPU baseline (one RHS; the headline's 4M points take minutes)")
+    ap.add_argument("--cpu-workload", default="ellipsoid_4m_x8",
+                    help="workload of the CPU baseline (one RHS, eval timed)")
     ap.add_argument("--serve", default="sphere8_60k,sphere16_250k_x64,sphere32_1m,ellipsoid_4m_x8",
                     help="comma-separated geometries for the serving leg ('' to skip)")
     ap.add_argument("--serve-qps", type=float, default=10.0)
@@ -392,7 +398,9 @@ def main():
         "vs_baseline": None,
         "dtype": {"mixed": "fp32+fp64 (FP32: translations with the HF solves folded in, M2L, near field, V list; "
                            "FP64: LF check-to-equivalent solves, coordinate differences)", "fp64": "fp64",
-                  "fp64_fused": "fp64 (fused solves)"}[args.precision],
+                  "fp64_fused": "fp64 (fused solves)",
+                  "mixed_split": "bf16x3+fp32+fp64 (translation GEMMs as three-term split-BF16 products "
+                                 "accumulated in FP32; otherwise as mixed)"}[args.precision],
         "data": "synthetic",
         "config": cfg,
         "roofline": head["roofline"],

@@ -122,9 +122,10 @@ struct ddfmm_ctx {
 template <class T>
 static int64_t vbytes(const std::vector<T>& v) { return (int64_t)v.size() * (int64_t)sizeof(T); }
 static void plan_bytes(const HostPlan& h, int precision, int64_t& fixed, int64_t& per_rhs, int64_t& transient) {
-    const bool mixed = precision == DDFMM_MIXED;
+    const bool mixed = precision == DDFMM_MIXED || precision == DDFMM_MIXED_SPLIT;
     fixed = vbytes(h.px) + vbytes(h.py) + vbytes(h.pz) + vbytes(h.perm) + vbytes(h.grid) +
             h.opmat_total * (int64_t)sizeof(double2) + (mixed ? h.opmat_total * (int64_t)sizeof(float2) : 0) +
+            (precision == DDFMM_MIXED_SPLIT ? h.opmat_total * 16 : 0) +
             vbytes(h.ktasks) + vbytes(h.ksrcs) + vbytes(h.groups) + vbytes(h.tiles) + vbytes(h.colB) +
             vbytes(h.colC) + vbytes(h.colA) + vbytes(h.fsrc) + vbytes(h.vtrg) + vbytes(h.vchunk) +
             vbytes(h.vsrc) + vbytes(h.vpk) + vbytes(h.rtrg) + (int64_t)sizeof(IoPtrs);
@@ -237,7 +238,8 @@ static int setup_common(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_
     if (p->ACCU < 1 || p->ACCU > 3) return fail(DDFMM_EINVAL, "ACCU must be 1, 2 or 3");
     if (p->ACCU != 2) return fail(DDFMM_EINVAL, "this build evaluates ACCU=2 (P=6) only");
     if (p->nrhs_max < 1) return fail(DDFMM_EINVAL, "nrhs_max must be >= 1");
-    if (p->precision != DDFMM_FP64 && p->precision != DDFMM_MIXED && p->precision != DDFMM_FP64_FUSED)
+    if (p->precision != DDFMM_FP64 && p->precision != DDFMM_MIXED && p->precision != DDFMM_FP64_FUSED &&
+        p->precision != DDFMM_MIXED_SPLIT)
         return fail(DDFMM_EINVAL, "unsupported precision mode");
     double lk = std::log2(p->K);
     if (!(p->K >= 1) || std::abs(lk - std::round(lk)) > 1e-12) return fail(DDFMM_EINVAL, "K must be a power of two >= 1");
@@ -288,7 +290,8 @@ static int setup_common(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_
         DevPlan& d = c->dp;
         d = DevPlan();
         d.N = h.N; d.P = h.P; d.NL = h.NL; d.ktype = c->prm.ktype;
-        d.mixed = c->precision == DDFMM_MIXED ? 1 : 0;     // FP64 and FP64_FUSED: FP64 arithmetic
+        // MIXED and MIXED_SPLIT: FP32 vectors; FP64 and FP64_FUSED: FP64 arithmetic
+        d.mixed = c->precision == DDFMM_MIXED || c->precision == DDFMM_MIXED_SPLIT ? 1 : 0;
         // the kernel-apply flavour is fixed per context (each right-hand side then has the
         // same bits alone and in any batch): the matrix-core kernel from four right-hand
         // sides up, the vector-ALU kernel below
@@ -331,6 +334,10 @@ static int setup_common(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_
         if (d.mixed) {
             d.opmat32 = c->alloc<float2>(h.opmat_total);
             HIPCHK(launch_to_f32(d.opmat, d.opmat32, h.opmat_total, c->stream));
+            if (c->precision == DDFMM_MIXED_SPLIT) {    // translations as split-BF16 products
+                d.opmatbf = (void*)c->alloc<double2>(h.opmat_total);
+                HIPCHK(launch_to_bf16s(d.opmat, d.opmatbf, h.opmat_total, c->stream));
+            }
             HIPCHK(hipStreamSynchronize(c->stream));
         }
         // work buffers, after the setup-time tables are gone

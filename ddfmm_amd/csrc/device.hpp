@@ -31,6 +31,7 @@ struct DevPlan {
     int* grid = nullptr;
     double2* opmat = nullptr;
     float2* opmat32 = nullptr;     // FP32 copy of the operator table (mixed mode)
+    void* opmatbf = nullptr;       // split-BF16 copy (16 bytes per entry; mixed-split mode)
     KTask* ktasks = nullptr;
     KSrc* ksrcs = nullptr;
     GGroup* groups = nullptr;
@@ -69,5 +70,6 @@ hipError_t direct_static_scratch(double2** out);
 hipError_t launch_kmatrix(const double* trg, int M, const double* src, int Ns, void* out, int ktype, int f32,
                           hipStream_t st);
 hipError_t launch_to_f32(const double2* in, float2* out, int64_t n, hipStream_t st);
+hipError_t launch_to_bf16s(const double2* in, void* out, int64_t n, hipStream_t st);
 
 }  // namespace ddfmm

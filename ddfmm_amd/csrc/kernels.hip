@@ -737,6 +737,305 @@ void launch_gemm(int mb, int n, int nrhs, hipStream_t st, const DevPlan& d, cons
 #undef DDFMM_GEMM_MB
 }
 
+// ---------------------------------------------------------------- split-BF16 complex GEMM (MFMA)
+// The translation GEMM of the mixed mode with every FP32 operand written as the sum of
+// three bf16 terms (x = x1 + x2 + x3, exact for FP32 x) and the products of
+// v_mfma_f32_16x16x32_bf16 kept down to the third order (x1y1, x1y2, x2y1, x1y3, x2y2,
+// x3y1; the dropped terms are below 2^-32 |x||y|), accumulated in FP32: FP32 accuracy at
+// 2.7x the FP32 matrix-core rate.  The operator table is split once at setup (Bf16s
+// entries); the vectors are split when a slab is staged.  A complex product is one real
+// product with K' = 32 per 16 complex k:
+//   Re C = [Ar, -Ai] [Br; Bi],   Im C = [Ai, Ar] [Br; Bi],
+// lane i + 16 g of a 16x16x32 block holding A[i][8g..8g+7] and B[8g..8g+7][i], so lanes
+// g < 2 take the real parts of k-octet g and lanes g >= 2 the imaginary parts of octet
+// g - 2.  Tiles, K segments, column gathers, wave skips and epilogue as in gemm_kernel.
+struct Bf16s {
+    unsigned short v[8];    // re1, im1, re2, im2, re3, im3, 0, 0
+};
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ unsigned short bf16_rne(float x) { return __builtin_bit_cast(unsigned short, (__bf16)x); }
+__device__ __forceinline__ float bf16_val(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+// x = h1 + h2 + h3 (each bf16, round to nearest even; the remainders are exact in FP32)
+__device__ __forceinline__ void split3(float x, unsigned short& h1, unsigned short& h2, unsigned short& h3) {
+    h1 = bf16_rne(x);
+    const float r1 = x - bf16_val(h1);
+    h2 = bf16_rne(r1);
+    h3 = bf16_rne(r1 - bf16_val(h2));
+}
+
+__global__ void to_bf16s_kernel(const double2* __restrict__ in, Bf16s* __restrict__ out, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        Bf16s e;
+        split3((float)in[i].x, e.v[0], e.v[2], e.v[4]);
+        split3((float)in[i].y, e.v[1], e.v[3], e.v[5]);
+        e.v[6] = e.v[7] = 0;
+        out[i] = e;
+    }
+}
+
+union Oct {                   // 8 bf16 of one plane and k-octet
+    uint4 u;
+    unsigned short h[8];
+    bf16x8 v;
+};
+
+template <int MB>
+__global__ __launch_bounds__(256) void gemm_bf16s_kernel(
+    const GGroup* __restrict__ groups, const GTile* __restrict__ tiles,
+    const int64_t* __restrict__ colB, const int64_t* __restrict__ colC, const int64_t* __restrict__ colA,
+    const Bf16s* __restrict__ opb, const float2* __restrict__ opm32, BufTable bt, int tile0, int nr) {
+    typedef float acc4 __attribute__((ext_vector_type(4)));
+    constexpr int KS = 16, TM = 16 * MB;
+    const int bx = xcd_item(blockIdx.x, gridDim.x);
+    GTile tl = tiles[tile0 + bx / nr];
+    const GGroup g = groups[tl.g];
+    const int ncx = g.ncols * nr;
+    tl.n0 = tl.n0 * nr + GEMM_TN * (bx % nr);
+    if (tl.n0 >= ncx) return;
+    const float2* B = bufp<float2>(bt, g.bbuf);
+    float2* C = bufp<float2>(bt, g.cbuf);
+    const int64_t sB = bt.stride[g.bbuf], sC = bt.stride[g.cbuf];
+    // [buffer][term][plane re/im][row or column][k-octet]: k-octet h of row r is stored
+    // at h ^ ((r >> 3) & 1), so the 16 lanes of a fragment read (16 rows, one octet) and
+    // the 16 lanes of a staging store cover all banks once
+    __shared__ uint4 As[2][3][2][TM][2];
+    __shared__ uint4 Bs[2][3][2][GEMM_TN][2];
+    __shared__ int64_t bo[MAXKB][GEMM_TN];
+    __shared__ int64_t ao[MAXKB];
+    __shared__ int used[MAXKB][4], act[MAXKB], wm[MAXKB], nact_s;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int kb = g.kb ? g.kb : g.K;
+    const int nkb = g.kb ? g.K / g.kb : 1;
+
+    // staging units: A (row am, k-octet ah) for tid < 2 TM; B (column bn, k-octet bh) for
+    // tid < 128
+    const int mrows = g.M - tl.m0;
+    const int64_t kstride = g.transA ? 1 : g.lda;
+    const bool hasA = tid < 2 * TM, hasB = tid < 2 * GEMM_TN;
+    const int am = tid % TM, ah = tid / TM;
+    const int64_t aoff = g.transA ? (int64_t)(tl.m0 + am) * g.lda : (int64_t)(tl.m0 + am);
+    const bool arow = hasA && am < mrows;
+    const int bn = tid & (GEMM_TN - 1), bh = tid >> 6;
+    int64_t bcol = -1;
+    const int ncol = tl.n0 + wave * 16 + li;
+    const int64_t coff = ncol < ncx ? colC[g.col0 + ncol / nr] + (int64_t)(ncol % nr) * sC : -1;
+
+    int nact = 1, wm0 = 0;
+    if (g.kb == 0) {
+        const int nv = ncx - tl.n0;
+        wm0 = nv >= GEMM_TN ? 15 : (1 << ((nv + 15) >> 4)) - 1;
+        const int n = tl.n0 + bn;
+        bcol = (hasB && n < ncx) ? colB[g.bcol0 + n / nr] + (int64_t)(n % nr) * sB : -1;
+    } else {
+        if (tid < MAXKB * 4) used[tid >> 2][tid & 3] = 0;
+        __syncthreads();
+        for (int e = tid; e < nkb * GEMM_TN; e += 256) {
+            const int h = e / GEMM_TN, j = e % GEMM_TN;
+            const int n = tl.n0 + j;
+            int64_t v = n < ncx ? colB[g.bcol0 + (int64_t)(n / nr) * nkb + h] : -1;
+            if (v >= 0) v += (int64_t)(n % nr) * sB;
+            bo[h][j] = v;
+            if (v >= 0) used[h][j >> 4] = 1;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int c = 0;
+            for (int h = 0; h < nkb; ++h) {
+                const int w = used[h][0] | (used[h][1] << 1) | (used[h][2] << 2) | (used[h][3] << 3);
+                if (w) {
+                    act[c] = h;
+                    wm[c] = w;
+                    ++c;
+                }
+            }
+            nact_s = c;
+        }
+        __syncthreads();
+        nact = nact_s;
+        if (tid < nact) ao[tid] = colA[g.acol0 + act[tid]];
+        __syncthreads();
+    }
+    Bf16s ra[8];
+    float2 rb[8];
+    auto load = [&](int i, int k0, bool newseg) {
+        const int kl = kb - k0;
+        if (newseg && hasB) bcol = bo[act[i]][bn];
+        if (hasA) {
+            const Bf16s* Ab = opb + (g.kb ? ao[i] : g.aoff) + k0 * kstride + aoff;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int k = 8 * ah + j;
+                Bf16s a = {};
+                if (arow && k < kl) a = Ab[k * kstride];
+                ra[j] = a;
+            }
+        }
+        if (hasB) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int k = 8 * bh + j;
+                float2 b = make_float2(0.f, 0.f);
+                if (bcol >= 0 && k < kl) b = B[bcol + k0 + k];
+                rb[j] = b;
+            }
+        }
+    };
+    auto store = [&](int q) {
+        if (hasA) {
+            const int sl = ah ^ ((am >> 3) & 1);
+#pragma unroll
+            for (int t = 0; t < 3; ++t)
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    Oct o;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) o.h[j] = ra[j].v[2 * t + c];
+                    As[q][t][c][am][sl] = o.u;
+                }
+        }
+        if (hasB) {
+            Oct o[3][2];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                split3(rb[j].x, o[0][0].h[j], o[1][0].h[j], o[2][0].h[j]);
+                split3(rb[j].y, o[0][1].h[j], o[1][1].h[j], o[2][1].h[j]);
+            }
+            const int sl = bh ^ ((bn >> 3) & 1);
+#pragma unroll
+            for (int t = 0; t < 3; ++t)
+#pragma unroll
+                for (int c = 0; c < 2; ++c) Bs[q][t][c][bn][sl] = o[t][c].u;
+        }
+    };
+
+    acc4 cr[MB], ci[MB];
+#pragma unroll
+    for (int a = 0; a < MB; ++a)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            cr[a][v] = 0;
+            ci[a][v] = 0;
+        }
+    // fragment addressing of this lane: octet lk & 1 (swizzled by row/column bit 3), plane
+    // re for lanes lk < 2 and im for lk >= 2 (B and the Re-C operand of A), im/re for the
+    // Im-C operand of A; the Re-C operand of lanes lk >= 2 is negated (-Ai)
+    const int oct = (lk & 1) ^ ((li >> 3) & 1);
+    const int px = lk >> 1, py = px ^ 1;
+    const unsigned neg = lk >= 2 ? 0x80008000u : 0u;
+    auto mfma_slab = [&](int i, int q) {
+        if (!(((g.kb ? wm[i] : wm0) >> wave) & 1)) return;
+        Oct bf[3];
+#pragma unroll
+        for (int t = 0; t < 3; ++t) bf[t].u = Bs[q][t][px][wave * 16 + li][oct];
+#pragma unroll
+        for (int a = 0; a < MB; ++a) {
+            Oct xr[3], yi[3];
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                xr[t].u = As[q][t][px][16 * a + li][oct];
+                xr[t].u.x ^= neg;
+                xr[t].u.y ^= neg;
+                xr[t].u.z ^= neg;
+                xr[t].u.w ^= neg;
+                yi[t].u = As[q][t][py][16 * a + li][oct];
+            }
+            // smallest terms first
+            cr[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xr[2].v, bf[0].v, cr[a], 0, 0, 0);
+            ci[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(yi[2].v, bf[0].v, ci[a], 0, 0, 0);
+            cr[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xr[1].v, bf[1].v, cr[a], 0, 0, 0);
+            ci[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(yi[1].v, bf[1].v, ci[a], 0, 0, 0);
+            cr[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xr[0].v, bf[2].v, cr[a], 0, 0, 0);
+            ci[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(yi[0].v, bf[2].v, ci[a], 0, 0, 0);
+            cr[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xr[1].v, bf[0].v, cr[a], 0, 0, 0);
+            ci[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(yi[1].v, bf[0].v, ci[a], 0, 0, 0);
+            cr[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xr[0].v, bf[1].v, cr[a], 0, 0, 0);
+            ci[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(yi[0].v, bf[1].v, ci[a], 0, 0, 0);
+            cr[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xr[0].v, bf[0].v, cr[a], 0, 0, 0);
+            ci[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(yi[0].v, bf[0].v, ci[a], 0, 0, 0);
+        }
+    };
+    auto next = [&](int i, int k0, int& ni, int& nk0) {
+        ni = i;
+        nk0 = k0 + KS;
+        if (nk0 >= kb) {
+            ni = i + 1;
+            nk0 = 0;
+        }
+    };
+    if (nact > 0) {
+        load(0, 0, g.kb != 0);
+        store(0);
+    }
+    // slab s+1 waits in registers while slab s runs; one barrier per slab
+    int i = 0, k0 = 0, ni, nk0;
+    next(i, k0, ni, nk0);
+    bool more = ni < nact;
+    if (more) load(ni, nk0, ni != i);
+    __syncthreads();
+    int q = 0;
+    while (i < nact) {
+        mfma_slab(i, q);
+        int n2i = nact, n2k0 = 0;
+        bool more2 = false;
+        if (more) {
+            store(q ^ 1);
+            next(ni, nk0, n2i, n2k0);
+            more2 = n2i < nact;
+            if (more2) load(n2i, n2k0, n2i != ni);
+        }
+        __syncthreads();
+        i = ni;
+        k0 = nk0;
+        ni = n2i;
+        nk0 = n2k0;
+        more = more2;
+        q ^= 1;
+    }
+    // D layout of the 16x16 output: lane holds rows 4 (lane / 16) + v, column lane % 16
+    if (coff < 0) return;
+    float2* Cc = C + coff;
+#pragma unroll
+    for (int a = 0; a < MB; ++a) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int m = tl.m0 + 16 * a + 4 * lk + v;
+            if (m >= g.M) continue;
+            float re = cr[a][v], im = ci[a][v];
+            if (g.soff >= 0) {
+                const float2 s = opm32[g.soff + m];
+                const float t = re * s.x - im * s.y;
+                im = re * s.y + im * s.x;
+                re = t;
+            }
+            if (g.mode == 0) {
+                Cc[m] = make_float2(re, im);
+            } else {
+                float2 o = Cc[m];
+                o.x += re;
+                o.y += im;
+                Cc[m] = o;
+            }
+        }
+    }
+}
+
+void launch_gemm_bf16s(int mb, int n, int nrhs, hipStream_t st, const DevPlan& d, int t0) {
+    const dim3 grid(n * nrhs, 1);
+#define DDFMM_GEMM_MB(X)                                                                                      \
+    case X:                                                                                                   \
+        gemm_bf16s_kernel<X><<<grid, 256, 0, st>>>(d.groups, d.tiles, d.colB, d.colC, d.colA,                 \
+                                                   (const Bf16s*)d.opmatbf, d.opmat32, d.bt, t0, nrhs);       \
+        break;
+    switch (mb) {
+        DDFMM_GEMM_MB(1) DDFMM_GEMM_MB(2) DDFMM_GEMM_MB(3) DDFMM_GEMM_MB(4) DDFMM_GEMM_MB(5)
+        DDFMM_GEMM_MB(6) DDFMM_GEMM_MB(7)
+    }
+#undef DDFMM_GEMM_MB
+}
+
 // ---------------------------------------------------------------- pruned (2P)^3 DFT, P = 6
 // The equivalent surface of a box sits on the 6^3 sub-cube [6, 12)^3 of the 12^3 grid
 // (grid index = round((x + W/2)/step) + P), so the forward transform has non-zero input
@@ -1339,6 +1638,8 @@ hipError_t launch_stage_t(const DevPlan& d, const Stage& s, int nrhs, hipStream_
         if (s.mb < 1 || s.mb > GEMM_MB) return hipErrorInvalidValue;
         if (d.mixed && s.prec == PREC_SOLVE)
             launch_gemm<double2, float2, double>(s.mb, n, nrhs, st, d, d.opmat, s.begin);
+        else if (d.mixed && d.opmatbf)
+            launch_gemm_bf16s(s.mb, n, nrhs, st, d, s.begin);
         else if (d.mixed)
             launch_gemm<float2, float2, float>(s.mb, n, nrhs, st, d, d.opmat32, s.begin);
         else
@@ -1418,6 +1719,11 @@ hipError_t launch_kgen(const KGen* jobs, int njobs, const double* gp, double2* o
         int n = std::min(65536, njobs - j0);
         kgen_kernel<<<n, 256, 0, st>>>(jobs, gp, opmat, j0, ktype);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_to_bf16s(const double2* in, void* out, int64_t n, hipStream_t st) {
+    if (n > 0) to_bf16s_kernel<<<grid_for(n, 256), 256, 0, st>>>(in, (Bf16s*)out, n);
     return hipGetLastError();
 }
 

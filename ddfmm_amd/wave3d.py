@@ -19,6 +19,7 @@ KERNEL_EXPR = 1
 FP64 = 0          # FP64, solves factored as in the reference
 MIXED = 1         # FP32 vectors/translations on MFMA, FP64 fused solves
 FP64_FUSED = 2    # FP64 with fused solves
+MIXED_SPLIT = 3   # MIXED with split-BF16 translation GEMMs
 
 
 class Kernel3d:

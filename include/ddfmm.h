@@ -31,8 +31,11 @@ enum { DDFMM_OK = 0, DDFMM_EINVAL = 1, DDFMM_ESETUP = 2, DDFMM_EDEVICE = 3, DDFM
 enum { DDFMM_KERNEL_HELM = 0, DDFMM_KERNEL_EXPR = 1 };   /* kernel3d.hpp:24-27 */
 /* FP64: all FP64, check->equivalent solves applied in the reference's factored form.
    MIXED: FP32 vectors and translations (MFMA f32), FP64 solves as precomputed matrices.
-   FP64_FUSED: FP64 everywhere with the MIXED mode's fused solve matrices. */
-enum { DDFMM_FP64 = 0, DDFMM_MIXED = 1, DDFMM_FP64_FUSED = 2 };
+   FP64_FUSED: FP64 everywhere with the MIXED mode's fused solve matrices.
+   MIXED_SPLIT: MIXED with the translation GEMMs as split-BF16 products (every FP32
+   operand the sum of three bf16 terms, products to third order on the bf16 matrix cores,
+   FP32 accumulation). */
+enum { DDFMM_FP64 = 0, DDFMM_MIXED = 1, DDFMM_FP64_FUSED = 2, DDFMM_MIXED_SPLIT = 3 };
 
 /* Wave3d / Mlib3d / Kernel3d options (wave3d_setup.cpp:29-61, mlib3d.cpp:70-91,
    tt.cpp:103-119): K, ctr, ACCU, NPQ, ptsmax, maxlevel, cell count per edge numC

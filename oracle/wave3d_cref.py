@@ -28,6 +28,9 @@ def lib():
         path = os.path.join(_HERE, "libwave3d_ref.so")
         if not os.path.exists(path):
             raise RuntimeError("oracle/libwave3d_ref.so missing: run `make -C oracle`")
+        # idle OpenMP workers sleep instead of spinning, so that numpy's BLAS threads in
+        # the same process (the numpy oracle, emulations) keep their cores
+        os.environ.setdefault("OMP_WAIT_POLICY", "passive")
         L = ctypes.CDLL(path)
         vp, ci, cd, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int64
         L.wr_error.restype = ctypes.c_char_p

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from ddfmm_amd import geometry
-from ddfmm_amd.wave3d import FP64, MIXED, Kernel3d, Mlib3d, Wave3d, direct_sum
+from ddfmm_amd.wave3d import FP64, MIXED, MIXED_SPLIT, Kernel3d, Mlib3d, Wave3d, direct_sum
 from oracle import wave3d_ref as R
 
 pytestmark = pytest.mark.gpu
@@ -120,7 +120,7 @@ def test_multi_rhs_eleven_mixed(oplib):
     w.close()
 
 
-@pytest.mark.parametrize("precision", [MIXED, FP64])
+@pytest.mark.parametrize("precision", [MIXED, FP64, MIXED_SPLIT])
 @pytest.mark.parametrize("ptsmax", [100, 400])
 def test_rhs_counts_keep_bits(oplib, precision, ptsmax):
     """Every kernel instance chosen by the right-hand-side count (kernel apply with 8 or 16
@@ -214,6 +214,32 @@ def test_mixed_precision_sphere60k(oplib):
     assert e64 < MIXED_VS_FP64_TOL
     assert e_direct < FMM_TOL
     wm.close()
+
+
+@pytest.mark.parametrize("nrhs_max", [1, 8])
+def test_mixed_split_sphere60k(oplib, nrhs_max):
+    """Split-BF16 translation GEMMs (three bf16 terms per FP32 operand, six products):
+    as close to the FP64 FMM as the FP32 translations (the products lose only terms below
+    2^-32), within 1e-5 of the direct sum, and each RHS keeps its bits in a batch."""
+    K, N = 8.0, 60000
+    pos = geometry.sphere(N, K, 0)
+    den = geometry.densities(N, 0, nrhs=2)
+    w = _wave(pos, K, oplib, nrhs_max=2)
+    v64 = w.eval(den)
+    w.close()
+    wm = _wave(pos, K, oplib, nrhs_max=nrhs_max, precision=MIXED)
+    vm = np.stack([wm.eval(den[r]) for r in range(2)])
+    wm.close()
+    ws = _wave(pos, K, oplib, nrhs_max=max(2, nrhs_max), precision=MIXED_SPLIT)
+    vs = ws.eval(den)
+    chk = geometry.check_targets(N, 1000, 0)
+    e_direct = rel(vs[:, chk], R.direct_sum(pos[chk], pos, den))
+    e64, em = rel(vs, v64), rel(vs, vm)
+    print(f"mixed_split sphere60k (nrhs_max {nrhs_max}): vs fp64 FMM {e64:.3e}, vs mixed {em:.3e}, "
+          f"mixed vs fp64 {rel(vm, v64):.3e}, vs direct {e_direct:.3e}", flush=True)
+    assert e64 < MIXED_VS_FP64_TOL and e_direct < FMM_TOL and em < 1e-6
+    assert np.array_equal(ws.eval(den[1]), vs[1])
+    ws.close()
 
 
 def test_mixed_multi_rhs_and_expr(oplib_expr):

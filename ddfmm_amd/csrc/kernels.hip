@@ -124,7 +124,7 @@ __global__ __launch_bounds__(KA_THREADS) void kapply_kernel(
     // grid-stride over the tasks (a grid smaller than the task count keeps a launch on the
     // second stream to a few blocks per CU); gridDim.x is a multiple of 8 or >= ntask
     for (int bi = blockIdx.x; bi < ntask; bi += gridDim.x) {
-    const KTask t = tasks[task0 + xcd_item(bi, ntask)];
+    const KTask t = tasks[task0 + bi];   // tasks are ordered by cost, longest first
     for (int tb = 0; tb < t.count; tb += KA_THREADS) {
         const int nt = min(KA_THREADS, t.count - tb);
         const int G = KA_THREADS / nt;
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(256) void kapply_mfma_kernel(
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int g = lane >> 4, li = lane & 15;
     for (int bi = blockIdx.x; bi < ntask; bi += gridDim.x) {
-    const KTask t = tasks[task0 + xcd_item(bi, ntask)];
+    const KTask t = tasks[task0 + bi];   // tasks are ordered by cost, longest first
     for (int tb = 0; tb < t.count; tb += 256) {
         const int nt = min(256, t.count - tb);
         const int nT = (nt + 15) >> 4;

@@ -108,9 +108,9 @@ struct VChunk {
     int u0, u1;
     int k0, k1;
 };
-constexpr int VCH_SRC = 400;
-constexpr int VCH_TRG = 256;
-constexpr int VCH_PK = 8192;
+constexpr int VCH_SRC = 200;
+constexpr int VCH_TRG = 128;
+constexpr int VCH_PK = 4096;
 
 // segmented reduction: out[out_off + m] (=|+=) sum_{j < cnt} BUF_RED[red_off + j * M + m]
 // (the contributions of one target are adjacent, in a fixed order: deterministic sums)

@@ -290,6 +290,26 @@ struct Builder {
         s.prec = (nm.find("uc2ue") != std::string::npos || nm.find("dc2de") != std::string::npos ||
                   nm.find("hf_up_e") != std::string::npos || nm.find("hf_dn_e") != std::string::npos)
                      ? PREC_SOLVE : PREC_TRANSLATION;
+        if (kind == ST_KAPPLY) {
+            // most expensive tasks first: the blocks are dispatched in task order, round robin
+            // over the XCDs, so the long leaves (up to 400 targets, up to 27 neighbours of
+            // as many points) start early and the launch ends on short ones.  Every task
+            // owns its target range, so the order does not change any result.
+            std::vector<std::pair<int64_t, int>> order;
+            for (int i = b; i < e; ++i) {
+                const KTask& t = pl.ktasks[i];
+                int64_t c = 0;
+                for (int q = t.s0; q < t.s1; ++q) c += pl.ksrcs[q].count;
+                order.push_back({-c * t.count, i});
+            }
+            std::stable_sort(order.begin(), order.end(),
+                             [](const std::pair<int64_t, int>& x, const std::pair<int64_t, int>& y) {
+                                 return x.first < y.first;
+                             });
+            std::vector<KTask> sorted;
+            for (auto& o : order) sorted.push_back(pl.ktasks[o.second]);
+            std::copy(sorted.begin(), sorted.end(), pl.ktasks.begin() + b);
+        }
         if (kind == ST_GEMM) {
             // the logical stage keeps its groups and its tiling, so that the tiles can be
             // rebuilt with another height limit (retile)

@@ -316,8 +316,9 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=sorted(PRECISIONS))
     ap.add_argument("--no-fp64-leg", action="store_true",
                     help="skip the plain-FP64 run of the headline workload (the mixed/FP64 ratio)")
-    ap.add_argument("--extra", default="sphere32_1m,sphere16_250k_x64",
-                    help="comma-separated further workloads timed in mixed and plain FP64 ('' to skip)")
+    ap.add_argument("--extra", default="sphere32_1m,sphere16_250k_x64,sphere16_250k_x64@1",
+                    help="comma-separated further workloads timed in mixed and plain FP64, '<name>@1' for the "
+                         "kernel exp(ikr) ('' to skip)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="one 256 MiB device-to-device copy (blit kernel) to calibrate PMC byte counters")
     ap.add_argument("--chk", type=int, default=1000)
@@ -345,10 +346,12 @@ def main():
     if not args.no_fp64_leg and args.precision != "fp64":
         fp64 = measure(args.workload, "fp64", args.kernel, args.nrhs, max(2, args.steps // 2), 1, args.chk, **tg)
     extra = {}
-    for wname in [x for x in args.extra.split(",") if x and x != args.workload]:
-        m = measure(wname, "mixed", args.kernel, None, max(2, args.steps // 2), 1, args.chk, **tg)
-        f = measure(wname, "fp64", args.kernel, None, max(2, args.steps // 2), 1, args.chk, **tg)
-        extra[wname] = {"nrhs": m["nrhs"], "n_points": m["n_points"], "mixed_ms": m["ms_per_step"],
+    for item in [x for x in args.extra.split(",") if x and x != args.workload]:
+        wname, _, kt = item.partition("@")      # "<workload>@1": kernel exp(ikr)
+        kt = int(kt) if kt else args.kernel
+        m = measure(wname, "mixed", kt, None, max(2, args.steps // 2), 1, args.chk, **tg)
+        f = measure(wname, "fp64", kt, None, max(2, args.steps // 2), 1, args.chk, **tg)
+        extra[item] = {"kernel": m["kernel"], "nrhs": m["nrhs"], "n_points": m["n_points"], "mixed_ms": m["ms_per_step"],
                         "mixed_value": m["value"], "mixed_rel_err": m["rel_err_vs_direct"],
                         "fp64_ms": f["ms_per_step"], "fp64_value": f["value"], "fp64_rel_err": f["rel_err_vs_direct"],
                         "mixed_vs_fp64": f["ms_per_step"] / m["ms_per_step"]}

@@ -142,6 +142,58 @@ void pack_by_mask(std::vector<std::pair<int, K>>& cols) {
 constexpr bool SECOND_STREAM_NEAR = true;   // near field (U list, direct W/X lists)
 constexpr bool SECOND_STREAM_LF = true;     // FFT, V list, X list per LF width
 
+// LF operators of one width (offsets into the point and operator tables)
+struct LfW {
+    const LowFreqEntry* le;
+    int r;
+    int64_t uep_p, ucp_p;          // kapply point offsets
+    int64_t pinv, V, invs, UH, tab;   // opmat offsets (fused solve or its factors)
+    int64_t ue2uc[8];              // generated
+    int grid;
+    bool has_children;
+};
+
+struct HfDir {          // one (width, direction) group
+    double W;
+    Index3 dir, srt, sgn, prm;
+    const HighFreqEntry* he;
+    int r1, r2, r1c;    // equivalent / check / child-equivalent sizes
+    int64_t uep_g, ucp_g, uepc_g;   // generator point offsets (shuffled)
+    int64_t ue2uc[8];   // generated matrices (relative offsets)
+    int64_t de2dc[8];   // W = 1 only: G(uep_LF(W/2) + s_c, -ucp) (not a transpose of ue2uc)
+    int64_t e1, e2, e3; // fused: e1 = E1 E2 E3 (r1 x r2); factored: E1, E2, E3
+    // fused mode: solves folded into the translations (relative generated offsets)
+    int64_t F = -1;     // F_c = (E1 E2 E3) ue2uc_c, r1 x r1c each, contiguous over c
+    int64_t D[8];       // W = 1: D_c = de2dc_c (E1 E2 E3)^T, r1c x r1
+    int64_t eT = -1;    // W = 1: (E1 E2 E3)^T (r2 x r1), uploaded
+    std::vector<BoxKey> out, in;
+};
+
+// E1..E3 (or their product) of one canonical HF entry
+struct EOff { int64_t e1, e2, eT; };
+// one reduction target of the HF downward pass: its contributions in adjacent slots
+struct RedT {
+    int64_t out_off;
+    int obuf, M, mode;
+    int cnt = 0, next = 0;
+    int64_t base = 0;
+};
+
+// W = 1 -> LF: incoming directions of one class, bundled by up to 8
+struct Bundle {
+    std::vector<const HfDir*> d;
+    std::map<BoxKey, int> cols;
+};
+
+// HF M2L pairs of one (W, centre offset)
+struct M2LGroup {
+    const HfDir* h = nullptr;
+    int64_t mat;
+    std::vector<int64_t> b;
+    std::vector<int> t;      // reduction target of each pair
+};
+
+
 struct Builder {
     const Params& prm;
     const Mlib& mlib;
@@ -372,6 +424,53 @@ struct Builder {
     }
 
     void run(const double* pos, int64_t N, bool want_dump);
+    // ---- planning state shared by the phases of run()
+    int P = 0, G = 0, G3 = 0;
+    std::map<double, std::vector<BoxKey>> ldmap;                                  // LF boxes per width
+    std::map<Index3, std::pair<std::vector<BoxKey>, std::vector<BoxKey>>> hdmap;  // HF out / in boxes per direction
+    std::map<BoxKey, int64_t> lfidx, ptoff;
+    std::vector<BoxKey> leaves;
+    std::map<double, LfW> lfw;
+    int NL = 0;
+    int64_t nlf = 0;
+    std::map<double, std::vector<HfDir>> hfl;
+    std::map<std::pair<BoxKey, Index3>, int64_t> outslot, inslot, oscr, iscr;
+    int64_t hf_up_size = 0, hf_dn_size = 0, scr_size = 0;
+    std::map<std::pair<double, Index3>, EOff> hf_entry_off;
+    std::vector<RedT> rts;
+    std::map<double, std::pair<int, int>> hf_region;   // W -> target range of the level
+    std::map<std::pair<BoxKey, Index3>, int> hf_rt;    // in-slot -> target
+    std::map<BoxKey, int> lf_rt;                       // LF child of a width-1 box -> target
+    std::vector<Bundle> w1b;
+    int lf_r0 = -1, lf_r1 = 0;
+    std::map<double, std::map<Index3, M2LGroup>> m2l;
+    int64_t red_size = 0, up_n = 0;
+    int64_t spec_max = 0, vtrg_max = 0;
+
+    int npts(const BoxKey& k) const { return (int)tree.boxes.at(k).ptidx.size(); }
+    int64_t lslot(const BoxKey& k) const { return lfidx.at(k) * NL; }
+    int64_t G_(int64_t rel) const { return rel + up_n; }     // generated-table offset -> table offset
+    int l2l_target(const HfDir& h, const BoxKey& ck);
+    bool child_present(const BoxKey& ck) const;
+    void assign(int r0, int r1);
+    int64_t slot(int ri);
+    void reduce_stage(int r0, int r1, const char* name);
+
+    void init(const double* pos, int64_t N);
+    void collect_boxes(const double* pos, int64_t N);
+    void lf_operators();
+    void hf_groups();
+    void hf_reductions();
+    void hf_m2l_pairs();
+    void assign_slots();
+    void operator_layout();
+    void buffer_sizes();
+    void stages_lf_upward();
+    void stages_hf_upward();
+    void stages_hf_downward();
+    void stages_lf_downward();
+    void stages_leaf();
+    void finish(bool want_dump);
 
     // Every descriptor's element range must lie inside the buffer or table it indexes: the
     // kernels trust these offsets, so a planner mistake is caught here on the host.
@@ -513,25 +612,78 @@ struct Builder {
     }
 };
 
-struct HfDir {          // one (width, direction) group
-    double W;
-    Index3 dir, srt, sgn, prm;
-    const HighFreqEntry* he;
-    int r1, r2, r1c;    // equivalent / check / child-equivalent sizes
-    int64_t uep_g, ucp_g, uepc_g;   // generator point offsets (shuffled)
-    int64_t ue2uc[8];   // generated matrices (relative offsets)
-    int64_t de2dc[8];   // W = 1 only: G(uep_LF(W/2) + s_c, -ucp) (not a transpose of ue2uc)
-    int64_t e1, e2, e3; // fused: e1 = E1 E2 E3 (r1 x r2); factored: E1, E2, E3
-    // fused mode: solves folded into the translations (relative generated offsets)
-    int64_t F = -1;     // F_c = (E1 E2 E3) ue2uc_c, r1 x r1c each, contiguous over c
-    int64_t D[8];       // W = 1: D_c = de2dc_c (E1 E2 E3)^T, r1c x r1
-    int64_t eT = -1;    // W = 1: (E1 E2 E3)^T (r2 x r1), uploaded
-    std::vector<BoxKey> out, in;
-};
 
 }  // namespace
 
+// The planning phases in order: each one reads the state the earlier ones left in the
+// Builder (boxes and lists, operator offsets, slot maps, reduction targets).
 void Builder::run(const double* pos, int64_t N, bool want_dump) {
+    init(pos, N);
+    collect_boxes(pos, N);
+    lf_operators();
+    hf_groups();
+    hf_reductions();
+    hf_m2l_pairs();
+    assign_slots();
+    operator_layout();
+    buffer_sizes();
+    stages_lf_upward();
+    stages_hf_upward();
+    stages_hf_downward();
+    stages_lf_downward();
+    stages_leaf();
+    finish(want_dump);
+}
+
+int Builder::l2l_target(const HfDir& h, const BoxKey& ck) {
+    if (h.W == 1.0) {
+        auto it = lf_rt.find(ck);
+        if (it != lf_rt.end()) return it->second;
+        RedT r;
+        r.out_off = lslot(ck); r.obuf = BUF_LF_CHK; r.M = NL; r.mode = 1;
+        lf_rt[ck] = (int)rts.size();
+        rts.push_back(r);
+        return (int)rts.size() - 1;
+    }
+    auto it = hf_rt.find({ck, parent_dir(h.dir)});
+    if (it == hf_rt.end()) throw Error("internal: child lacks inherited direction");
+    return it->second;
+}
+
+bool Builder::child_present(const BoxKey& ck) const {
+    auto bi = tree.boxes.find(ck);
+    return bi != tree.boxes.end() && bi->second.has_pts();
+}
+
+void Builder::assign(int r0, int r1) {
+    int64_t cur = 0;
+    for (int i = r0; i < r1; ++i) {
+        rts[i].base = cur;
+        cur += (int64_t)rts[i].cnt * rts[i].M;
+    }
+    red_size = std::max(red_size, cur);
+}
+
+int64_t Builder::slot(int ri) {
+    RedT& r = rts[ri];
+    if (r.next >= r.cnt) throw Error("internal: reduction slot overflow");
+    return r.base + (int64_t)(r.next++) * r.M;
+}
+
+void Builder::reduce_stage(int r0, int r1, const char* name) {
+    int q0 = (int)pl.rtrg.size();
+    for (int i = r0; i < r1; ++i) {
+        const RedT& r = rts[i];
+        if (r.next != r.cnt) throw Error("internal: reduction target missing contributions");
+        if (r.mode == 1 && r.cnt == 0) continue;
+        RTarget t{};
+        t.out_off = r.out_off; t.red_off = r.base; t.cnt = r.cnt; t.M = r.M; t.obuf = r.obuf; t.mode = r.mode;
+        pl.rtrg.push_back(t);
+    }
+    stage(ST_REDUCE, q0, (int)pl.rtrg.size(), name);
+}
+
+void Builder::init(const double* pos, int64_t N) {
     tree.prm = prm;
     tree.pos = pos;
     tree.N = N;
@@ -539,14 +691,12 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
     pl.prm = prm;
     pl.N = N;
     pl.P = prm.ACCU == 1 ? 4 : (prm.ACCU == 2 ? 6 : 8);
-    const int P = pl.P, G = 2 * P, G3 = G * G * G;
-    auto& boxes = tree.boxes;
+    P = pl.P; G = 2 * P; G3 = G * G * G;
+}
 
+void Builder::collect_boxes(const double* pos, int64_t N) {
+    auto& boxes = tree.boxes;
     // ---------------- live boxes, leaves, LF index (ConstructMaps, eval 252-295)
-    std::map<double, std::vector<BoxKey>> ldmap;
-    std::map<Index3, std::pair<std::vector<BoxKey>, std::vector<BoxKey>>> hdmap;
-    std::map<BoxKey, int64_t> lfidx, ptoff;
-    std::vector<BoxKey> leaves;
     for (auto& kv : boxes) {
         const Box& b = kv.second;
         if (!b.has_pts()) continue;
@@ -571,24 +721,15 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
         pl.py[i] = pos[3 * (size_t)pl.perm[i] + 1];
         pl.pz[i] = pos[3 * (size_t)pl.perm[i] + 2];
     }
-    auto npts = [&](const BoxKey& k) { return (int)boxes.at(k).ptidx.size(); };
     pl.stats.npts = N;
     pl.stats.nleaf = (int64_t)leaves.size();
     pl.stats.nlf = (int64_t)lfidx.size();
     pl.stats.nlevels_lf = (int)ldmap.size();
+}
 
+void Builder::lf_operators() {
+    auto& boxes = tree.boxes;
     // ---------------- LF operators per width
-    struct LfW {
-        const LowFreqEntry* le;
-        int r;
-        int64_t uep_p, ucp_p;          // kapply point offsets
-        int64_t pinv, V, invs, UH, tab;   // opmat offsets (fused solve or its factors)
-        int64_t ue2uc[8];              // generated
-        int grid;
-        bool has_children;
-    };
-    std::map<double, LfW> lfw;
-    int NL = 0;
     for (auto& kv : ldmap) {
         double W = kv.first;
         LfW w;
@@ -643,15 +784,11 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
         lfw[W] = w;
     }
     pl.NL = NL;
-    const int64_t nlf = (int64_t)lfidx.size();
-    auto lslot = [&](const BoxKey& k) { return lfidx.at(k) * NL; };
+    nlf = (int64_t)lfidx.size();
+}
 
+void Builder::hf_groups() {
     // ---------------- HF groups per (W, dir)
-    std::map<double, std::vector<HfDir>> hfl;
-    std::map<std::pair<BoxKey, Index3>, int64_t> outslot, inslot, oscr, iscr;
-    int64_t hf_up_size = 0, hf_dn_size = 0, scr_size = 0;
-    struct EOff { int64_t e1, e2, eT; };
-    std::map<std::pair<double, Index3>, EOff> hf_entry_off;   // E1..E3 (or their product) per entry
     for (auto& kv : hdmap) {
         const Index3& d = kv.first;
         double W = double(d.linfty() / prm.NPQ);
@@ -759,7 +896,10 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
         }
         scr_size = std::max(scr_size, std::max(lvl_o, lvl_i));
     }
+}
 
+void Builder::hf_reductions() {
+    auto& boxes = tree.boxes;
     // ---------------- reduction targets of the HF downward pass
     // Every in-slot (box, direction) of a level receives its M2L contributions (one per
     // far-field source box) and its L2L contributions (one per direction of the parent that
@@ -767,16 +907,6 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
     // per incoming direction of the parent.  Each contribution is written by a plain store
     // into its own slot of BUF_RED (the slots of one target adjacent), then one reduction per
     // level sums them into the target: no atomics, fixed summation order.
-    struct RedT {
-        int64_t out_off;
-        int obuf, M, mode;
-        int cnt = 0, next = 0;
-        int64_t base = 0;
-    };
-    std::vector<RedT> rts;
-    std::map<double, std::pair<int, int>> hf_region;          // W -> target range of the level
-    std::map<std::pair<BoxKey, Index3>, int> hf_rt;           // in-slot -> target
-    std::map<BoxKey, int> lf_rt;                              // LF child of a width-1 box -> target
     for (auto lt = hfl.rbegin(); lt != hfl.rend(); ++lt) {
         int r0 = (int)rts.size();
         for (auto& h : lt->second)
@@ -789,34 +919,11 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
         hf_region[lt->first] = {r0, (int)rts.size()};
     }
     // L2L contributions, enumerated in the order the L2L groups are emitted below
-    auto l2l_target = [&](const HfDir& h, const BoxKey& ck) -> int {
-        if (h.W == 1.0) {
-            auto it = lf_rt.find(ck);
-            if (it != lf_rt.end()) return it->second;
-            RedT r;
-            r.out_off = lslot(ck); r.obuf = BUF_LF_CHK; r.M = NL; r.mode = 1;
-            lf_rt[ck] = (int)rts.size();
-            rts.push_back(r);
-            return (int)rts.size() - 1;
-        }
-        auto it = hf_rt.find({ck, parent_dir(h.dir)});
-        if (it == hf_rt.end()) throw Error("internal: child lacks inherited direction");
-        return it->second;
-    };
-    auto child_present = [&](const BoxKey& ck) {
-        auto bi = boxes.find(ck);
-        return bi != boxes.end() && bi->second.has_pts();
-    };
     // Width 1 -> LF children: the incoming directions of one class (equal solve size, so
     // equal K blocks) are bundled by up to 8 into one K-blocked GEMM per (bundle, child
     // position) whose columns are the boxes with any of the bundle's directions: one LF
     // check contribution per bundle instead of one per direction.  Columns: the union of
     // the bundle's in-lists, each with its direction-presence mask.
-    struct Bundle {
-        std::vector<const HfDir*> d;
-        std::map<BoxKey, int> cols;
-    };
-    std::vector<Bundle> w1b;
     if (hfl.count(1.0)) {
         std::map<int, std::vector<const HfDir*>> cls;
         for (auto& h : hfl.at(1.0)) cls[prm.fuse_solves ? h.r1 : h.r2].push_back(&h);
@@ -831,7 +938,6 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
                 w1b.push_back(std::move(b));
             }
     }
-    int lf_r0 = -1;
     for (auto lt = hfl.rbegin(); lt != hfl.rend(); ++lt) {
         if (lt->first == 1.0) {
             lf_r0 = (int)rts.size();
@@ -850,16 +956,12 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
                     if (child_present(ck)) rts[l2l_target(h, ck)].cnt++;
                 }
     }
-    const int lf_r1 = (int)rts.size();
+    lf_r1 = (int)rts.size();
+}
 
+void Builder::hf_m2l_pairs() {
+    auto& boxes = tree.boxes;
     // ---------------- HF M2L pairs grouped by (W, offset) (HighFrequencyM2L, eval 876-928)
-    struct M2LGroup {
-        const HfDir* h;
-        int64_t mat;
-        std::vector<int64_t> b;
-        std::vector<int> t;      // reduction target of each pair
-    };
-    std::map<double, std::map<Index3, M2LGroup>> m2l;
     for (auto& lv : hfl) {
         double W = lv.first;
         auto& groups = m2l[W];
@@ -893,42 +995,19 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
             }
         }
     }
+}
 
+void Builder::assign_slots() {
     // slots: each region (one HF level, or the LF children of width 1) starts at 0
-    int64_t red_size = 0;
-    auto assign = [&](int r0, int r1) {
-        int64_t cur = 0;
-        for (int i = r0; i < r1; ++i) {
-            rts[i].base = cur;
-            cur += (int64_t)rts[i].cnt * rts[i].M;
-        }
-        red_size = std::max(red_size, cur);
-    };
     for (auto& rg : hf_region) assign(rg.second.first, rg.second.second);
     if (lf_r0 >= 0) assign(lf_r0, lf_r1);
-    auto slot = [&](int ri) {
-        RedT& r = rts[ri];
-        if (r.next >= r.cnt) throw Error("internal: reduction slot overflow");
-        return r.base + (int64_t)(r.next++) * r.M;
-    };
-    auto reduce_stage = [&](int r0, int r1, const char* name) {
-        int q0 = (int)pl.rtrg.size();
-        for (int i = r0; i < r1; ++i) {
-            const RedT& r = rts[i];
-            if (r.next != r.cnt) throw Error("internal: reduction target missing contributions");
-            if (r.mode == 1 && r.cnt == 0) continue;
-            RTarget t{};
-            t.out_off = r.out_off; t.red_off = r.base; t.cnt = r.cnt; t.M = r.M; t.obuf = r.obuf; t.mode = r.mode;
-            pl.rtrg.push_back(t);
-        }
-        stage(ST_REDUCE, q0, (int)pl.rtrg.size(), name);
-    };
+}
 
+void Builder::operator_layout() {
     // ---------------- operator table layout: uploaded part, then generated part
-    const int64_t up_n = (int64_t)pl.opmat.size();
+    up_n = (int64_t)pl.opmat.size();
     pl.opmat_total = up_n + gen_cursor;
     for (auto& k : pl.kgen) k.out += up_n;
-    auto G_ = [&](int64_t rel) { return rel + up_n; };
     pl.stats.opmat_elems = pl.opmat_total;
     pl.stats.opmat_generated = gen_cursor;
 
@@ -971,10 +1050,12 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
             push_gemm_stages(pl.setup_stages, s);
         }
     }
+}
 
+void Builder::buffer_sizes() {
     // ---------------- buffers (elements per rhs)
-    pl.bufsize[BUF_DEN] = N;
-    pl.bufsize[BUF_VAL] = N;
+    pl.bufsize[BUF_DEN] = pl.N;
+    pl.bufsize[BUF_VAL] = pl.N;
     pl.bufsize[BUF_LF_UPEQ] = nlf * NL;
     pl.bufsize[BUF_LF_CHK] = nlf * NL;
     pl.bufsize[BUF_LF_TMP] = prm.fuse_solves ? 0 : nlf * NL;
@@ -984,7 +1065,10 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
     pl.bufsize[BUF_HF_A] = prm.fuse_solves ? 0 : scr_size;
     pl.bufsize[BUF_HF_B] = prm.fuse_solves ? 0 : scr_size;
     pl.bufsize[BUF_RED] = red_size;
+}
 
+void Builder::stages_lf_upward() {
+    auto& boxes = tree.boxes;
     // ================= stages
     cur_sec = 0;
     stage(ST_GATHER, 0, 0, "gather");
@@ -1063,7 +1147,10 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
     }
     stage(ST_ZERO, 0, 0, "zero_lfchk", BUF_LF_CHK, pl.bufsize[BUF_LF_CHK]);
     cur_sec = 4;
+}
 
+void Builder::stages_hf_upward() {
+    auto& boxes = tree.boxes;
     // ---- HF upward (W = 1, 2, ...), EvalUpwardHigh 773-854
     for (auto& lv : hfl) {
         double W = lv.first;
@@ -1128,7 +1215,9 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
             }
         }
     }
+}
 
+void Builder::stages_hf_downward() {
     // ---- HF downward (W = max .. 1), EvalDownwardHigh 996-1040
     cur_sec = 5;
     for (auto lt = hfl.rbegin(); lt != hfl.rend(); ++lt) {
@@ -1239,9 +1328,11 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
             cur_sec = 5;
         }
     }
+}
 
+void Builder::stages_lf_downward() {
+    auto& boxes = tree.boxes;
     // ---- LF downward (W descending), EvalDownwardLow 493-583
-    int64_t spec_max = 0, vtrg_max = 0;
     for (auto lt = ldmap.rbegin(); lt != ldmap.rend(); ++lt) {
         double W = lt->first;
         LfW& w = lfw[W];
@@ -1422,7 +1513,10 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
     }
     pl.bufsize[BUF_SPEC] = spec_max * G3;
     pl.bufsize[BUF_TSPEC] = vtrg_max * G3;
+}
 
+void Builder::stages_leaf() {
+    auto& boxes = tree.boxes;
     // ---- leaf kernel (eval 550-560, 585-718), in two launches: the near field (U list,
     // W and X lists evaluated directly from point densities) needs only the densities and
     // runs on the second stream from the start; the far part (W list through source
@@ -1480,7 +1574,10 @@ void Builder::run(const double* pos, int64_t N, bool want_dump) {
     leaf_tasks(false);
     stage(ST_SCATTER, 0, 0, "scatter");
     merge_sections();
+}
 
+void Builder::finish(bool want_dump) {
+    auto& boxes = tree.boxes;
     // ---------------- per-stage algorithmic work (per RHS)
     for (auto& st : pl.stages) stage_work(pl, st);
 

@@ -287,16 +287,24 @@ def stage_detail(w, st, dd, vd, nrhs, precision, workload, kernel=0):
     roof["launches_per_eval"] = launches
     roof["avg_launch_us"] = t_dom / max(launches, 1) * 1e6
     roof["work"] = work_desc
-    # the same stages one at a time on one stream: each time is the stage's own
+    # the same stages one at a time on one stream: each time is the stage's own, and the
+    # dominant class's launches run with nothing beside them (frac_alone)
     w.set_timing(2)
     serial = {}
+    t_alone = 0.0
     for _ in range(reps):
         w.eval_device(dd, vd, nrhs)
         for name, ms, work, kind, prec in w.stage_times():
             if kind in (ST_FORK, ST_JOIN):
                 continue
             serial[name] = serial.get(name, 0.0) + ms / reps
+            if kernel_class(kind, prec) == dom:
+                t_alone += ms * 1e-3 / reps
     w.set_timing(False)
+    if t_alone > 0:
+        roof["achieved_alone"] = roof["achieved"] * t_dom / t_alone     # same work, its own time
+        roof["frac_alone"] = roof["achieved_alone"] / roof["peak"]
+        roof["avg_launch_us_alone"] = t_alone / max(launches, 1) * 1e6
     log("stages alone (ms): " + ", ".join(f"{n}={v:.3f}" for n, v in sorted(serial.items(), key=lambda kv: -kv[1])[:14])
         + f"; sum {sum(serial.values()):.2f}")
     return {"roofline": roof, "per_kernel_ms": {k: round(v["ms"], 4) for k, v in per.items()},

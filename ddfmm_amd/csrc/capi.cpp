@@ -135,6 +135,13 @@ static void plan_bytes(const HostPlan& h, int precision, int64_t& fixed, int64_t
     transient = vbytes(h.genpts) + vbytes(h.kgen);
 }
 
+// The kernel-apply flavour is fixed per context (each right-hand side then has the same
+// bits alone and in any batch): the matrix-core kernel in the mixed modes from four
+// right-hand sides up, the vector-ALU kernel otherwise.
+static bool kapply_on_mfma(int precision, int nrhs_max) {
+    return (precision == DDFMM_MIXED || precision == DDFMM_MIXED_SPLIT) && nrhs_max >= 4;
+}
+
 static std::string gib(int64_t b) {
     char t[32];
     std::snprintf(t, sizeof(t), "%.3f GiB", (double)b / (1 << 30));
@@ -257,6 +264,11 @@ static int setup_common(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_
         q.numC = p->numC; q.ktype = p->kernel;
         q.fuse_solves = p->precision != DDFMM_FP64;
         q.nrhs_max = p->nrhs_max;
+        // the matrix-core kernel apply (mixed modes, four or more right-hand sides)
+        // evaluates near-field leaf pairs of a small leaf once for both directions; with
+        // leaves of up to 128 points most pairs qualify (with larger leaves few do, and the
+        // reduction of the mirror products costs more than it saves)
+        q.near_pairs = kapply_on_mfma(p->precision, p->nrhs_max) && p->ptsmax <= 2 * MIRROR_MAX_TARGETS;
         c->nrhs_max = p->nrhs_max;
         c->precision = p->precision;
         std::shared_ptr<const Mlib> mlib = shared ? shared : load_mlib(ldfile, hdfile);
@@ -292,10 +304,7 @@ static int setup_common(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_
         d.N = h.N; d.P = h.P; d.NL = h.NL; d.ktype = c->prm.ktype;
         // MIXED and MIXED_SPLIT: FP32 vectors; FP64 and FP64_FUSED: FP64 arithmetic
         d.mixed = c->precision == DDFMM_MIXED || c->precision == DDFMM_MIXED_SPLIT ? 1 : 0;
-        // the kernel-apply flavour is fixed per context (each right-hand side then has the
-        // same bits alone and in any batch): the matrix-core kernel from four right-hand
-        // sides up, the vector-ALU kernel below
-        d.kmfma = d.mixed && c->nrhs_max >= 4 ? 1 : 0;
+        d.kmfma = kapply_on_mfma(c->precision, c->nrhs_max) ? 1 : 0;
         const int64_t esz = d.mixed ? (int64_t)sizeof(float2) : (int64_t)sizeof(double2);
         d.px = c->upload(h.px); d.py = c->upload(h.py); d.pz = c->upload(h.pz);
         d.perm = c->upload(h.perm);

@@ -55,6 +55,8 @@ struct Params {
     int ACCU = 2, NPQ = 4, ptsmax = 100, maxlevel = 12, numC = 1, ktype = 0;
     bool fuse_solves = false;   // apply check->equivalent solves as single precomputed matrices
     int nrhs_max = 1;           // right-hand sides an evaluation may carry (GEMM launch sizing)
+    bool near_pairs = false;    // near field: evaluate each pair of leaves that are in each
+                                // other's near lists once, for both directions
 };
 
 class Error : public std::runtime_error {

@@ -7,8 +7,9 @@
 //   gemm_kernel    grouped complex GEMM on MFMA (16x16x4, f64 or f32) over (box, rhs)
 //                  columns with gather/scatter and store/add epilogues: every translation
 //                  the reference applies as a per-box zgemv (vecmatop.cpp:102-125)
-//   reduce_kernel  fixed-order sum of the per-contribution slots of the HF M2L/L2L
-//                  targets (the reference's zgemv accumulations into one vector)
+//   reduce_kernel  fixed-order sum of per-contribution slots: the HF M2L/L2L targets
+//                  (the reference's zgemv accumulations into one vector) and the
+//                  near-field mirror products of pairs evaluated once
 //   fft_fwd_kernel / vlist_acc_kernel / vlist_inv_kernel
 //                  FFT-diagonal LF M2L on the (2P)^3 grid
 //                  (wave3d_eval.cpp:602-661)
@@ -272,8 +273,144 @@ __device__ __forceinline__ void kmfma_steps(kacc4 (&acc)[4][NB], const double (&
     }
 }
 
-template <int NB, int KT>
-__global__ __launch_bounds__(256) void kapply_mfma_kernel(
+// Orders one wave's LDS writes before its later LDS reads (and those reads before later
+// writes): LDS executes a wave's instructions in order, so only the compiler must not move
+// the accesses across.
+__device__ __forceinline__ void wave_lds_order() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Near-field pairs evaluated once for both directions.  For a mirrored source leaf s of the
+// task's leaf k (KSrc::moff >= 0) one wave takes 16-source tiles st = st0, st0 + 4, ...,
+// each lane reading its four sources from memory (no block staging: the waves share no
+// sources, so they run without barriers), and evaluates G over its row tiles m x the
+// tile's 16 sources, four values per lane and row tile in the forward operand layout
+// (lane i + 16 g: target i, sources 4 jj + g):
+//   forward  acc[m] += G v_s                      (as kmfma_steps, groups 4 st + jj)
+//   mirror   macc   += G^T den_k over the targets (K = targets 4 j + g of row tile m)
+// The mirror's A operand is the transposed tile (lane i + 16 g: source i, targets 4 j + g),
+// taken from per-wave LDS copies of two row tiles at a time.  G of all row tiles comes
+// first and the products of different row tiles interleave (independent accumulators;
+// the mirror sums even and odd row tiles apart and adds the two at the end).  The task has
+// at most four row tiles and this wave holds them all (the waves split the sources), so
+// macc is the tile's whole mirror: stored into the source's slot.
+template <int NM, int NB, int KT>
+__device__ __forceinline__ void kmfma_mirror(kacc4 (&acc)[4][NB], const double (&tx)[4], const double (&ty)[4],
+                                             const double (&tz)[4], const double* px, const double* py,
+                                             const double* pz, const float2* vb, int64_t vs,
+                                             const float2 (*tv)[8 * NB + 1], float2* gt, int n, int st0, int g,
+                                             int li, float* mslot, int64_t ms, int r0, int nr) {
+    const bool imcol = li & 1;
+    const int nst = (n + 15) >> 4;
+    for (int st = st0; st < nst; st += 4) {
+        float gr[NM][4], gi[NM][4];
+        float b1[4][NB], b2[4][NB];
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+            // sources past n: coordinates 0 (a finite G) and zero densities
+            const int j = 16 * st + 4 * jj + g;
+            const bool in = j < n;
+            const double xs = in ? px[j] : 0.0, ys = in ? py[j] : 0.0, zs = in ? pz[j] : 0.0;
+#pragma unroll
+            for (int c = 0; c < NB; ++c) {
+                const int q = 8 * c + (li >> 1);
+                const float2 v = in && q < nr ? vb[(int64_t)(r0 + q) * vs + j] : make_float2(0.f, 0.f);
+                b1[jj][c] = imcol ? v.y : v.x;
+                b2[jj][c] = imcol ? v.x : -v.y;
+            }
+#pragma unroll
+            for (int m = 0; m < NM; ++m) {
+                const float2 gv = green32(tx[m] - xs, ty[m] - ys, tz[m] - zs, KT);
+                gr[m][jj] = gv.x;
+                gi[m][jj] = gv.y;
+            }
+        }
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+            for (int m = 0; m < NM; ++m)
+#pragma unroll
+                for (int c = 0; c < NB; ++c) acc[m][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(gr[m][jj], b1[jj][c], acc[m][c], 0, 0, 0);
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+            for (int m = 0; m < NM; ++m)
+#pragma unroll
+                for (int c = 0; c < NB; ++c) acc[m][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(gi[m][jj], b2[jj][c], acc[m][c], 0, 0, 0);
+        kacc4 macc[2][NB];
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int c = 0; c < NB; ++c) macc[p][c] = kacc4{0, 0, 0, 0};
+#pragma unroll
+        for (int h = 0; h < (NM + 1) / 2; ++h) {
+            constexpr int T = 16 * 17;       // one tile [source][target], rows padded to 17
+            const int mh = NM - 2 * h < 2 ? NM - 2 * h : 2;
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                if (p >= mh) break;
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj)
+                    gt[p * T + (4 * jj + g) * 17 + li] = make_float2(gr[2 * h + p][jj], gi[2 * h + p][jj]);
+            }
+            wave_lds_order();
+            float hr[2][4], hi[2][4], d1[2][4][NB], d2[2][4][NB];
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                if (p >= mh) break;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const float2 e = gt[p * T + li * 17 + 4 * j + g];
+                    hr[p][j] = e.x;
+                    hi[p][j] = e.y;
+#pragma unroll
+                    for (int c = 0; c < NB; ++c) {
+                        const float2 w = tv[16 * (2 * h + p) + 4 * j + g][8 * c + (li >> 1)];
+                        d1[p][j][c] = imcol ? w.y : w.x;
+                        d2[p][j][c] = imcol ? w.x : -w.y;
+                    }
+                }
+            }
+            wave_lds_order();
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int p = 0; p < 2; ++p) {
+                    if (p >= mh) break;
+#pragma unroll
+                    for (int c = 0; c < NB; ++c)
+                        macc[p][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(hr[p][j], d1[p][j][c], macc[p][c], 0, 0, 0);
+                }
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int p = 0; p < 2; ++p) {
+                    if (p >= mh) break;
+#pragma unroll
+                    for (int c = 0; c < NB; ++c)
+                        macc[p][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(hi[p][j], d2[p][j][c], macc[p][c], 0, 0, 0);
+                }
+        }
+        // rows 4 g + v: sources 16 st + 4 g + v; column li: (rhs 8 c + li / 2, re/im)
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            const int q = 8 * c + (li >> 1);
+            if (q >= nr) continue;
+            float* o = mslot + 2 * ((int64_t)(r0 + q) * ms) + (li & 1);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int jx = 16 * st + 4 * g + v;
+                if (jx >= n) continue;
+                o[2 * jx] = macc[0][c][v] + macc[1][c][v];
+            }
+        }
+    }
+}
+
+// (the 8-RHS mirror instance fits 168 VGPRs without spills: three waves per SIMD)
+template <int NB, int KT, bool MIR>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIR && NB == 1 ? 3 : 1))) void kapply_mfma_kernel(
     const KTask* __restrict__ tasks, const KSrc* __restrict__ srcs,
     const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
     BufTable bt, int task0, int ntask, int nrhs) {
@@ -285,6 +422,11 @@ __global__ __launch_bounds__(256) void kapply_mfma_kernel(
     __shared__ __align__(16) float2 sv[KM_ST][NR + 1];
     static_assert(sizeof(float2) * KM_ST * (NR + 1) >= sizeof(float) * 4 * 4 * 64 * 4 * NB, "reduction fits the tile");
     float* red = (float*)&sv[0][0];
+    // mirror mode: the densities of a task that evaluates mirrored sources (at most 64
+    // targets, one pass: every wave holds all its row tiles; rows past the task zero), the
+    // mirror's operand, and one transposition tile per wave
+    __shared__ __align__(16) float2 tv[MIR ? MIRROR_MAX_TARGETS : 1][NR + 1];
+    __shared__ float2 gts[MIR ? 4 : 1][2 * 16 * 17];
     const int r0 = blockIdx.y * NR;
     const int nr = min(NR, nrhs - r0);
     const int tid = threadIdx.x, lane = tid & 63;
@@ -317,10 +459,36 @@ __global__ __launch_bounds__(256) void kapply_mfma_kernel(
         for (int m = 0; m < 4; ++m)
 #pragma unroll
             for (int c = 0; c < NB; ++c) acc[m][c] = kacc4{0, 0, 0, 0};
+        if (MIR && t.count <= MIRROR_MAX_TARGETS) {    // block-uniform
+            const float2* den = bufp<float2>(bt, BUF_DEN) + t.off;
+            const int64_t ds = bt.stride[BUF_DEN];
+            for (int e = tid; e < MIRROR_MAX_TARGETS * NR; e += 256) {
+                const int i = e / NR, q = e % NR;
+                tv[i][q] = (i < nt && q < nr) ? den[(int64_t)(r0 + q) * ds + i] : make_float2(0.f, 0.f);
+            }
+            __syncthreads();
+        }
         for (int si = t.s0; si < t.s1; ++si) {
             const KSrc s = srcs[si];
             const float2* vb = bufp<float2>(bt, s.vbuf) + s.voff;
             const int64_t vs = bt.stride[s.vbuf];
+            if (MIR && s.moff >= 0) {
+                // the first tile rotates with the source, which spreads the tiles evenly
+                // over the waves
+                const int st0 = (wave + si) & 3;
+                float* ms = (float*)(bufp<float2>(bt, BUF_NMIR) + s.moff);
+                const int64_t mst = bt.stride[BUF_NMIR];
+                float2* gt = gts[MIR ? wave : 0];
+                const double *qx = px + s.off, *qy = py + s.off, *qz = pz + s.off;
+                switch (nm) {      // wave-uniform; every wave holds the task's row tiles
+                case 4: kmfma_mirror<4, NB, KT>(acc, tx, ty, tz, qx, qy, qz, vb, vs, tv, gt, s.count, st0, g, li, ms, mst, r0, nr); break;
+                case 3: kmfma_mirror<3, NB, KT>(acc, tx, ty, tz, qx, qy, qz, vb, vs, tv, gt, s.count, st0, g, li, ms, mst, r0, nr); break;
+                case 2: kmfma_mirror<2, NB, KT>(acc, tx, ty, tz, qx, qy, qz, vb, vs, tv, gt, s.count, st0, g, li, ms, mst, r0, nr); break;
+                case 1: kmfma_mirror<1, NB, KT>(acc, tx, ty, tz, qx, qy, qz, vb, vs, tv, gt, s.count, st0, g, li, ms, mst, r0, nr); break;
+                default: break;
+                }
+                continue;
+            }
             for (int sb = 0; sb < s.count; sb += KM_ST) {
                 const int n = min(KM_ST, s.count - sb);
                 __syncthreads();
@@ -1564,12 +1732,13 @@ __device__ __forceinline__ void reduce_rows(const V* __restrict__ in, V* __restr
 }
 
 template <class V>
-__global__ __launch_bounds__(256) void reduce_kernel(const RTarget* __restrict__ rtrg, int n, BufTable bt, int t0) {
+__global__ __launch_bounds__(256) void reduce_kernel(const RTarget* __restrict__ rtrg, int n, BufTable bt, int t0,
+                                                     const V* __restrict__ inb, int64_t ins) {
     const int ti = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (ti >= n) return;
     const RTarget r = rtrg[t0 + ti];
     const int rhs = blockIdx.y;
-    const V* in = bufp<V>(bt, BUF_RED) + (int64_t)rhs * bt.stride[BUF_RED] + r.red_off;
+    const V* in = inb + (int64_t)rhs * ins + r.red_off;
     V* out = bufp<V>(bt, r.obuf) + (int64_t)rhs * bt.stride[r.obuf] + r.out_off;
     if (r.M <= 128) {
         reduce_rows<V, 2>(in, out, r.M, r.cnt, r.mode, 0, lane);
@@ -1609,19 +1778,23 @@ hipError_t launch_stage_t(const DevPlan& d, const Stage& s, int nrhs, hipStream_
         if constexpr (sizeof(V) == 8) if (d.kmfma) {   // mixed precision: the products on the matrix cores
             const int nb = nrhs > 8 ? 2 : 1;
             const dim3 grid(gx, (nrhs + 8 * nb - 1) / (8 * nb));
-            if (nb == 1) {
-                if (d.ktype == 0)
-                    kapply_mfma_kernel<1, 0><<<grid, 256, 0, st>>>(d.ktasks, d.ksrcs, d.px, d.py, d.pz, d.bt, s.begin, n, nrhs);
-                else
-                    kapply_mfma_kernel<1, 1><<<grid, 256, 0, st>>>(d.ktasks, d.ksrcs, d.px, d.py, d.pz, d.bt, s.begin, n, nrhs);
-            } else {
-                if (d.ktype == 0)
-                    kapply_mfma_kernel<2, 0><<<grid, 256, 0, st>>>(d.ktasks, d.ksrcs, d.px, d.py, d.pz, d.bt, s.begin, n, nrhs);
-                else
-                    kapply_mfma_kernel<2, 1><<<grid, 256, 0, st>>>(d.ktasks, d.ksrcs, d.px, d.py, d.pz, d.bt, s.begin, n, nrhs);
+#define DDFMM_KMFMA(NB, KT, MIR) \
+    kapply_mfma_kernel<NB, KT, MIR><<<grid, 256, 0, st>>>(d.ktasks, d.ksrcs, d.px, d.py, d.pz, d.bt, s.begin, n, nrhs)
+            const int sel = (nb == 2 ? 4 : 0) + (d.ktype ? 2 : 0) + (s.mb == 1 ? 1 : 0);
+            switch (sel) {
+            case 0: DDFMM_KMFMA(1, 0, false); break;
+            case 1: DDFMM_KMFMA(1, 0, true); break;
+            case 2: DDFMM_KMFMA(1, 1, false); break;
+            case 3: DDFMM_KMFMA(1, 1, true); break;
+            case 4: DDFMM_KMFMA(2, 0, false); break;
+            case 5: DDFMM_KMFMA(2, 0, true); break;
+            case 6: DDFMM_KMFMA(2, 1, false); break;
+            case 7: DDFMM_KMFMA(2, 1, true); break;
             }
+#undef DDFMM_KMFMA
             break;
         }
+        if (s.mb == 1) return hipErrorInvalidValue;    // mirror slots need the matrix-core kernel
         if (nrhs == 1) {
             kapply_kernel<V, 1><<<dim3(gx, 1), KA_THREADS, 0, st>>>(d.ktasks, d.ksrcs, d.px, d.py, d.pz, d.bt,
                                                                     s.begin, n, nrhs, d.ktype);
@@ -1683,7 +1856,9 @@ hipError_t launch_stage_t(const DevPlan& d, const Stage& s, int nrhs, hipStream_
         break;
     }
     case ST_REDUCE:
-        reduce_kernel<V><<<dim3((n + 3) / 4, nrhs), 256, 0, st>>>(d.rtrg, n, d.bt, s.begin);
+        if (s.buf != BUF_RED && s.buf != BUF_NMIR) return hipErrorInvalidValue;
+        reduce_kernel<V><<<dim3((n + 3) / 4, nrhs), 256, 0, st>>>(d.rtrg, n, d.bt, s.begin, (const V*)d.bt.p[s.buf],
+                                                                  d.bt.stride[s.buf]);
         break;
     default:
         return hipErrorInvalidValue;

@@ -23,16 +23,27 @@ enum Buf : int {
     BUF_SPEC,       // LF V-list source spectra of one level, (2P)^3 each
     BUF_RED,        // HF M2L/L2L contributions of one level, one slot per (target, source)
     BUF_TSPEC,      // LF V-list target spectra of one level, (2P)^3 each, in V-target order
+    BUF_NMIR,       // near-field mirror products, one slot per pair evaluated both ways (the
+                    // slots into one leaf adjacent, in source-task order)
     BUF_COUNT
 };
 // setup-time GEMMs read and write the operator table itself (FP64), as buffer BUF_OPMAT
 constexpr int BUF_OPMAT = BUF_COUNT;
 
+// near-field pairs evaluated once for both directions: the evaluating leaf has at most this
+// many points (one pass of four row tiles of 16: each wave holds every target)
+constexpr int MIRROR_MAX_TARGETS = 64;
+
 // kernel-apply: out(target_i) (+)= sum_src sum_j G(t_i, s_j) v_j
 // point coordinates: pts[off + i] * scale + c
+// moff >= 0 (near field, pairs of leaves evaluated once for both directions): the source
+// is a leaf whose own task skips this task's leaf; the kernel also writes
+// sum_i G(t_i, s_j) den(t_i) into BUF_NMIR[moff + j], which a reduction adds to the
+// source leaf's potentials
 struct KSrc {
     int64_t off;
     int64_t voff;
+    int64_t moff = -1;
     double c[3];
     double scale;
     int count;
@@ -112,8 +123,10 @@ constexpr int VCH_SRC = 200;
 constexpr int VCH_TRG = 128;
 constexpr int VCH_PK = 4096;
 
-// segmented reduction: out[out_off + m] (=|+=) sum_{j < cnt} BUF_RED[red_off + j * M + m]
-// (the contributions of one target are adjacent, in a fixed order: deterministic sums)
+// segmented reduction: out[out_off + m] (=|+=) sum_{j < cnt} in[red_off + j * M + m], in =
+// the stage's buffer (BUF_RED: HF M2L/L2L contributions; BUF_NMIR: near-field mirror
+// products); the contributions of one target are adjacent, in a fixed order: deterministic
+// sums
 struct RTarget {
     int64_t out_off;
     int64_t red_off;
@@ -147,13 +160,15 @@ enum StagePrec : int { PREC_TRANSLATION = 0, PREC_SOLVE = 1 };
 struct Stage {
     int kind;
     int begin, end;     // descriptor range (tasks / tiles / sources / targets)
-    int buf;            // ST_ZERO: buffer id; ST_FORK/ST_JOIN: event index; ST_VLIST: first chunk
+    int buf;            // ST_ZERO: buffer id; ST_FORK/ST_JOIN: event index; ST_VLIST: first chunk;
+                        // ST_REDUCE: the contributions' buffer
     int64_t count;      // ST_ZERO: elements per rhs; ST_VLIST: chunks
     int prec;           // StagePrec
     int stream;         // 0: the evaluation (caller's) stream, 1: the concurrent second stream
     double work;        // per RHS: GEMM useful flops, KAPPLY interactions, FFT spectra, V-list pairs,
                         // reduction elements read
-    int mb;             // ST_GEMM: row blocks of every tile in the range (one kernel instance)
+    int mb;             // ST_GEMM: row blocks of every tile in the range (one kernel instance);
+                        // ST_KAPPLY: 1 when sources carry mirror slots (KSrc::moff)
     int lstage;         // ST_GEMM: logical stage (all tile heights of one planned GEMM stage), else -1
     double exec;        // ST_GEMM: MFMA flops per RHS the kernel executes (padding included)
     char name[24];

@@ -347,11 +347,15 @@ struct Builder {
             // over the XCDs, so the long leaves (up to 400 targets, up to 27 neighbours of
             // as many points) start early and the launch ends on short ones.  Every task
             // owns its target range, so the order does not change any result.
+            // (a source evaluated for both directions costs about 5/4 of a one-way source)
             std::vector<std::pair<int64_t, int>> order;
             for (int i = b; i < e; ++i) {
                 const KTask& t = pl.ktasks[i];
                 int64_t c = 0;
-                for (int q = t.s0; q < t.s1; ++q) c += pl.ksrcs[q].count;
+                for (int q = t.s0; q < t.s1; ++q) {
+                    c += pl.ksrcs[q].count * (pl.ksrcs[q].moff >= 0 ? 5 : 4);
+                    if (pl.ksrcs[q].moff >= 0) s.mb = 1;
+                }
                 order.push_back({-c * t.count, i});
             }
             std::stable_sort(order.begin(), order.end(),
@@ -587,6 +591,15 @@ struct Builder {
                 need(k.off >= 0 && k.off + k.count <= npt, "kapply source points");
                 need(k.vbuf >= 0 && k.vbuf < BUF_COUNT && k.voff >= 0 && k.voff + k.count <= pl.bufsize[k.vbuf],
                      "kapply source values");
+                // a mirrored source: point sources read unscaled, and the task's own
+                // densities are the mirror's operand
+                need(k.moff == -1 || (k.moff >= 0 && k.moff + k.count <= pl.bufsize[BUF_NMIR] &&
+                                      t.count <= MIRROR_MAX_TARGETS &&
+                                      k.scale == 1.0 && k.c[0] == 0 && k.c[1] == 0 && k.c[2] == 0 &&
+                                      t.obuf == BUF_VAL && t.ooff == t.off && t.scale == 1.0 &&
+                                      t.c[0] == 0 && t.c[1] == 0 && t.c[2] == 0 &&
+                                      t.off + t.count <= pl.bufsize[BUF_DEN]),
+                     "kapply mirror slot");
             }
         }
         for (const FftSrc& f : pl.fsrc) {
@@ -604,10 +617,16 @@ struct Builder {
                 need(pl.vpair[q].offidx >= 0 && pl.vpair[q].offidx < 343, "vlist offset");
             }
         }
-        for (const RTarget& r : pl.rtrg) {
-            need(r.obuf >= 0 && r.obuf < BUF_COUNT && r.out_off >= 0 && r.out_off + r.M <= pl.bufsize[r.obuf],
-                 "reduce out");
-            need(r.cnt >= 0 && r.red_off >= 0 && r.red_off + (int64_t)r.cnt * r.M <= pl.bufsize[BUF_RED], "reduce in");
+        for (const Stage& st : pl.stages) {
+            if (st.kind != ST_REDUCE) continue;
+            need(st.buf == BUF_RED || st.buf == BUF_NMIR, "reduce input buffer");
+            for (int q = st.begin; q < st.end; ++q) {
+                const RTarget& r = pl.rtrg[q];
+                need(r.obuf >= 0 && r.obuf < BUF_COUNT && r.out_off >= 0 && r.out_off + r.M <= pl.bufsize[r.obuf],
+                     "reduce out");
+                need(r.cnt >= 0 && r.red_off >= 0 && r.red_off + (int64_t)r.cnt * r.M <= pl.bufsize[st.buf],
+                     "reduce in");
+            }
         }
     }
 };
@@ -680,7 +699,7 @@ void Builder::reduce_stage(int r0, int r1, const char* name) {
         t.out_off = r.out_off; t.red_off = r.base; t.cnt = r.cnt; t.M = r.M; t.obuf = r.obuf; t.mode = r.mode;
         pl.rtrg.push_back(t);
     }
-    stage(ST_REDUCE, q0, (int)pl.rtrg.size(), name);
+    stage(ST_REDUCE, q0, (int)pl.rtrg.size(), name, BUF_RED);
 }
 
 void Builder::init(const double* pos, int64_t N) {
@@ -1521,9 +1540,62 @@ void Builder::stages_leaf() {
     // W and X lists evaluated directly from point densities) needs only the densities and
     // runs on the second stream from the start; the far part (W list through source
     // equivalent densities, L2T) is added at the end
+    // Near-field pairs evaluated once for both directions (prm.near_pairs): leaves that are
+    // each in the other's near list exactly once and the smaller of which (the lower task
+    // index on a tie) has at most MIRROR_MAX_TARGETS points.  That leaf evaluates the pair in
+    // one target pass; larger tasks keep their source tiles shared by all waves (a mirrored
+    // pass of a large leaf would re-read every source per 64 targets).  Evaluator i writes
+    // the mirror product into slot (j, rank of i among j's evaluators); task j skips i; a
+    // reduction adds j's slots to its potentials in that order.
+    const std::vector<BoxKey> order = morton_sorted(leaves);
+    std::map<BoxKey, int> leaf_index;
+    for (int i = 0; i < (int)order.size(); ++i) leaf_index[order[i]] = i;
+    auto near_list = [&](const BoxKey& k) {
+        const Box& b = boxes.at(k);
+        std::vector<BoxKey> out(b.U.begin(), b.U.end());
+        for (auto& s : b.W)
+            if (boxes.at(s).terminal() && npts(s) < NL) out.push_back(s);
+        if (npts(k) < NL) out.insert(out.end(), b.X.begin(), b.X.end());
+        return out;
+    };
+    std::map<std::pair<int, int>, int> pair_count;
+    std::vector<std::vector<int>> mirror_in(order.size());   // j -> its evaluators, ascending
+    std::vector<int64_t> mirror_base(order.size(), 0);
+    auto evaluates = [&](int i, int j) {      // i evaluates the mutual pair (i, j)
+        const int64_t ni = npts(order[i]), nj = npts(order[j]);
+        return ni <= MIRROR_MAX_TARGETS && (ni < nj || (ni == nj && i < j));
+    };
+    if (prm.near_pairs) {
+        for (int i = 0; i < (int)order.size(); ++i)
+            for (auto& s : near_list(order[i])) {
+                auto it = leaf_index.find(s);
+                if (it != leaf_index.end()) ++pair_count[{i, it->second}];
+            }
+        for (auto& pc : pair_count) {      // in (i, j) order: each mirror_in[j] ascends
+            const int i = pc.first.first, j = pc.first.second;
+            if (i != j && pc.second == 1 && evaluates(i, j)) {
+                auto r = pair_count.find({j, i});
+                if (r != pair_count.end() && r->second == 1) mirror_in[j].push_back(i);
+            }
+        }
+        int64_t cur = 0;
+        for (int j = 0; j < (int)order.size(); ++j) {
+            mirror_base[j] = cur;
+            cur += (int64_t)mirror_in[j].size() * npts(order[j]);
+            pl.stats.near_mirrored += (int64_t)mirror_in[j].size();
+        }
+        pl.bufsize[BUF_NMIR] = cur;
+    }
+    auto mirrored = [&](int i, int j) {      // (i, j) is a mutual pair evaluated once
+        if (!prm.near_pairs || i == j) return false;
+        const int e = evaluates(i, j) ? i : evaluates(j, i) ? j : -1, o = e == i ? j : i;
+        if (e < 0) return false;
+        const auto& in = mirror_in[o];
+        return std::binary_search(in.begin(), in.end(), e);
+    };
     auto leaf_tasks = [&](bool near) {
         int t0 = (int)pl.ktasks.size();
-        for (auto& k : morton_sorted(leaves)) {
+        for (auto& k : order) {
             const Box& b = boxes.at(k);
             double W = tree.width(k);
             LfW& w = lfw[W];
@@ -1538,13 +1610,23 @@ void Builder::stages_leaf() {
                 pl.stats.near_interactions += (int64_t)npts(k) * npts(s);
             };
             if (near) {
-                for (auto& s : b.U) direct(s);
-                for (auto& s : b.W) {
-                    const Box& sb = boxes.at(s);
-                    if (sb.terminal() && npts(s) < NL) direct(s);
+                const int i = leaf_index.at(k);
+                for (auto& s : near_list(k)) {
+                    auto it = leaf_index.find(s);
+                    const int j = it == leaf_index.end() ? -1 : it->second;
+                    if (j >= 0 && mirrored(i, j)) {
+                        if (!evaluates(i, j)) {               // task j evaluates the pair
+                            pl.stats.near_interactions += (int64_t)npts(k) * npts(s);
+                            continue;
+                        }
+                        const auto& in = mirror_in[j];
+                        const int64_t rank = std::lower_bound(in.begin(), in.end(), i) - in.begin();
+                        direct(s);
+                        pl.ksrcs.back().moff = mirror_base[j] + rank * npts(s);
+                    } else {
+                        direct(s);
+                    }
                 }
-                if (npts(k) < NL)
-                    for (auto& s : b.X) direct(s);
             } else {
                 for (auto& s : b.W) {
                     const Box& sb = boxes.at(s);
@@ -1566,10 +1648,41 @@ void Builder::stages_leaf() {
             t.s1 = (int)pl.ksrcs.size();
             pl.ktasks.push_back(t);
         }
-        stage(ST_KAPPLY, t0, (int)pl.ktasks.size(), near ? "leaf_near" : "leaf_far");
+        if (near && prm.near_pairs) {
+            // two launches: the tasks that evaluate mirrored pairs (kernel instance with the
+            // mirror path and its registers) and the others (the leaner instance)
+            std::stable_partition(pl.ktasks.begin() + t0, pl.ktasks.end(), [&](const KTask& t) {
+                for (int q = t.s0; q < t.s1; ++q)
+                    if (pl.ksrcs[q].moff >= 0) return false;
+                return true;
+            });
+            int t1 = t0;
+            while (t1 < (int)pl.ktasks.size()) {
+                const KTask& t = pl.ktasks[t1];
+                bool mir = false;
+                for (int q = t.s0; q < t.s1 && !mir; ++q) mir = pl.ksrcs[q].moff >= 0;
+                if (mir) break;
+                ++t1;
+            }
+            stage(ST_KAPPLY, t0, t1, "leaf_near");
+            stage(ST_KAPPLY, t1, (int)pl.ktasks.size(), "leaf_near_pairs");
+        } else {
+            stage(ST_KAPPLY, t0, (int)pl.ktasks.size(), near ? "leaf_near" : "leaf_far");
+        }
     };
     cur_sec = 3;
     leaf_tasks(true);
+    if (prm.near_pairs) {      // the mirror products into their leaves, sources in task order
+        const int q0 = (int)pl.rtrg.size();
+        for (int j = 0; j < (int)order.size(); ++j) {
+            if (mirror_in[j].empty()) continue;
+            RTarget t{};
+            t.out_off = ptoff.at(order[j]); t.red_off = mirror_base[j]; t.cnt = (int)mirror_in[j].size();
+            t.M = (int)npts(order[j]); t.obuf = BUF_VAL; t.mode = 1;
+            pl.rtrg.push_back(t);
+        }
+        stage(ST_REDUCE, q0, (int)pl.rtrg.size(), "near_mirror", BUF_NMIR);
+    }
     cur_sec = 6;
     leaf_tasks(false);
     stage(ST_SCATTER, 0, 0, "scatter");

@@ -14,6 +14,7 @@ namespace ddfmm {
 struct PlanStats {
     int64_t npts = 0, nleaf = 0, nlf = 0, nhf_out = 0, nhf_in = 0;
     int64_t near_interactions = 0;      // sum over U/W/X-direct pairs of n_t * n_s (leaf kernel, point sources)
+    int64_t near_mirrored = 0;          // near-field leaf pairs evaluated once for both directions
     int64_t leaf_op_interactions = 0;   // leaf kernel interactions with operator point sets (L2T, W via upeq)
     int64_t s2m_interactions = 0, x2chk_interactions = 0;
     int64_t v_pairs = 0, v_targets = 0, v_sources = 0;

@@ -45,7 +45,8 @@ void stage_work(const HostPlan& pl, Stage& s) {
     } else if (s.kind == ST_KAPPLY) {
         for (int t = s.begin; t < s.end; ++t) {
             const KTask& k = pl.ktasks[t];
-            for (int q = k.s0; q < k.s1; ++q) w += (double)k.count * pl.ksrcs[q].count;
+            for (int q = k.s0; q < k.s1; ++q)   // a mirrored source: both directions
+                w += (double)k.count * pl.ksrcs[q].count * (pl.ksrcs[q].moff >= 0 ? 2 : 1);
         }
     } else if (s.kind == ST_FFTFWD) {
         w = s.end - s.begin;

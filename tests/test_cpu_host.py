@@ -81,8 +81,8 @@ def test_eval_without_device_is_rejected(oplib):
         _lib.lib().ddfmm_destroy(h)
 
 
-def _host_stages(oplib, pos, K, precision):
-    prm = _lib.Params(K=K, ACCU=2, NPQ=4, ptsmax=100, maxlevel=12, numC=1, kernel=0, nrhs_max=1,
+def _host_stages(oplib, pos, K, precision, nrhs_max=1, ptsmax=100, dump=False):
+    prm = _lib.Params(K=K, ACCU=2, NPQ=4, ptsmax=ptsmax, maxlevel=12, numC=1, kernel=0, nrhs_max=nrhs_max,
                       precision=precision)
     h = ctypes.c_void_p()
     _lib.check(_lib.lib().ddfmm_create(-1, ctypes.byref(h)))
@@ -98,8 +98,15 @@ def _host_stages(oplib, pos, K, precision):
         _lib.check(_lib.lib().ddfmm_stage_times(h, n, names, None, work, kinds, ctypes.byref(cnt)))
         st = _lib.Stats()
         _lib.check(_lib.lib().ddfmm_get_stats(h, ctypes.byref(st)))
-        return ([(names.raw[24 * i:24 * i + 24].split(b"\0")[0].decode(), work[i], kinds[i] % 16, kinds[i] // 16)
-                 for i in range(n)], st.as_dict())
+        out = ([(names.raw[24 * i:24 * i + 24].split(b"\0")[0].decode(), work[i], kinds[i] % 16, kinds[i] // 16)
+                for i in range(n)], st.as_dict())
+        if dump:
+            ln = ctypes.c_int64()
+            _lib.check(_lib.lib().ddfmm_dump_lists(h, None, 0, ctypes.byref(ln)))
+            buf = ctypes.create_string_buffer(ln.value + 1)
+            _lib.check(_lib.lib().ddfmm_dump_lists(h, buf, ln.value, ctypes.byref(ln)))
+            out += (buf.raw[:ln.value].decode(),)
+        return out
     finally:
         _lib.lib().ddfmm_destroy(h)
 
@@ -126,6 +133,53 @@ def test_stage_precision_classes_and_work(oplib, precision):
         assert {"lf_uc2ue_1", "lf_uc2ue_2", "hf_up_e3", "hf_dn_e1t"} <= names
     total = sum(s[1] for s in gemm)
     assert abs(total - st["gemm_flops"]) <= 1e-9 * st["gemm_flops"]
+
+
+def test_near_pairs_cover_every_interaction(oplib):
+    """Mixed contexts on the matrix-core kernel apply (4 or more RHS, leaves of at most 128
+    points) evaluate a near-field pair of leaves once for both directions when each is in
+    the other's near list (U, W-direct, X-direct) and the smaller has at most 64 points.
+    Checked against the list dump: the near-field launches still count every interaction
+    once per direction, the mirror reduction adds one slot per such pair, of the larger
+    leaf's size, and the one-way plan (fewer RHS) has neither."""
+    a = geometry.sphere(20000, 3.4, 1)
+    b = geometry.sphere(80000, 3.4, 2)
+    pos = np.ascontiguousarray(np.concatenate([a, b[b[:, 2] > 1.2]]))
+    one, st1 = _host_stages(oplib, pos, 4.0, 1, nrhs_max=1)
+    two, st2, dump = _host_stages(oplib, pos, 4.0, 1, nrhs_max=8, dump=True)
+    ST_KAPPLY, ST_REDUCE = 2, 7
+    work = lambda stages, names: sum(w for n, w, k, p in stages if n in names)
+    names1, names2 = [s[0] for s in one], [s[0] for s in two]
+    assert "leaf_near_pairs" not in names1 and "near_mirror" not in names1
+    assert "leaf_near_pairs" in names2 and "near_mirror" in names2
+    assert st1["near_interactions"] == st2["near_interactions"] > 0
+    assert work(one, {"leaf_near"}) == st1["near_interactions"]
+    assert work(two, {"leaf_near", "leaf_near_pairs"}) == st2["near_interactions"]
+    for other in ("leaf_far", "lf_s2m", "lf_x2chk"):
+        assert work(one, {other}) == work(two, {other}), other
+    assert [s[2] for s in two if s[0] == "near_mirror"] == [ST_REDUCE]
+    # the pairs from the list dump: boxes (tag bit 2 terminal, points), their near lists
+    NL = 152
+    box = {}
+    for ln in dump.splitlines():
+        m = re.match(r"^(\S+) t(\d+) n(\d+) f\d+ U\[([^\]]*)\] V\[[^\]]*\] W\[([^\]]*)\] X\[([^\]]*)\]", ln)
+        if m:
+            box[m[1]] = (int(m[2]) & 2 != 0, int(m[3]), m[4].split(), m[5].split(), m[6].split())
+    cnt = {}
+    for k, (term, n, U, W, X) in box.items():
+        if not term or n == 0:
+            continue
+        near = list(U) + [s for s in W if box[s][0] and box[s][1] < NL] + (list(X) if n < NL else [])
+        for s in near:
+            cnt[(k, s)] = cnt.get((k, s), 0) + 1
+    expect = 0
+    for (k, s), c in cnt.items():
+        if k < s and c == 1 and cnt.get((s, k)) == 1:
+            nk, ns = box[k][1], box[s][1]
+            if min(nk, ns) <= 64:
+                expect += max(nk, ns)
+    assert expect > 0
+    assert work(two, {"near_mirror"}) == expect
 
 
 def test_blocked_gemm_padding_stays_small(oplib):

@@ -69,11 +69,11 @@ __device__ __forceinline__ double2 green64(double dx, double dy, double dz, int 
     return make_double2(c, s);
 }
 
-// FP32 G(r) from FP64 differences: 1/r from v_rsq_f32, r = r^2 / r, and v_sin_f32 /
-// v_cos_f32 on the phase r - rint(r) in revolutions (three transcendental operations).
+// FP32 G(r) from FP64 differences: r^2 in FP64 (full-rate FMAs, one conversion), 1/r from
+// v_rsq_f32, r = r^2 / r, and v_sin_f32 / v_cos_f32 on the phase r - rint(r) in
+// revolutions (three transcendental operations).
 __device__ __forceinline__ float2 green32(double dx, double dy, double dz, int ktype) {
-    float fx = (float)dx, fy = (float)dy, fz = (float)dz;
-    float r2 = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
+    float r2 = (float)fma(dz, dz, fma(dy, dy, dx * dx));
     r2 = r2 < 1e-16f ? 1.0f : r2;
     const float ir = __builtin_amdgcn_rsqf(r2);
     // phase r - rint(r) with the product r2 * ir unrounded

@@ -123,7 +123,7 @@ template <class T>
 static int64_t vbytes(const std::vector<T>& v) { return (int64_t)v.size() * (int64_t)sizeof(T); }
 static void plan_bytes(const HostPlan& h, int precision, int64_t& fixed, int64_t& per_rhs, int64_t& transient) {
     const bool mixed = precision == DDFMM_MIXED || precision == DDFMM_MIXED_SPLIT;
-    fixed = vbytes(h.px) + vbytes(h.py) + vbytes(h.pz) + vbytes(h.perm) + vbytes(h.grid) +
+    fixed = vbytes(h.px) + vbytes(h.py) + vbytes(h.pz) + 2 * vbytes(h.perm) + vbytes(h.grid) +
             h.opmat_total * (int64_t)sizeof(double2) + (mixed ? h.opmat_total * (int64_t)sizeof(float2) : 0) +
             (precision == DDFMM_MIXED_SPLIT ? h.opmat_total * 16 : 0) +
             vbytes(h.ktasks) + vbytes(h.ksrcs) + vbytes(h.groups) + vbytes(h.tiles) + vbytes(h.colB) +
@@ -308,6 +308,11 @@ static int setup_common(ddfmm_ctx* c, const double* pos, int64_t n, const ddfmm_
         const int64_t esz = d.mixed ? (int64_t)sizeof(float2) : (int64_t)sizeof(double2);
         d.px = c->upload(h.px); d.py = c->upload(h.py); d.pz = c->upload(h.pz);
         d.perm = c->upload(h.perm);
+        {
+            std::vector<int> iperm(h.perm.size());
+            for (size_t i = 0; i < h.perm.size(); ++i) iperm[h.perm[i]] = (int)i;
+            d.iperm = c->upload(iperm);
+        }
         d.grid = c->upload(h.grid);
         d.opmat = c->alloc<double2>(h.opmat_total);
         if (!h.opmat.empty())

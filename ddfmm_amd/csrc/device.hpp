@@ -27,7 +27,8 @@ struct DevPlan {
     int mixed = 0;                 // 1: FP32 vectors/translations, FP64 solves
     int kmfma = 0;                 // mixed and nrhs_max >= 4: kernel applies on the matrix cores
     double *px = nullptr, *py = nullptr, *pz = nullptr;
-    int* perm = nullptr;
+    int* perm = nullptr;           // sorted position -> input index
+    int* iperm = nullptr;          // input index -> sorted position
     int* grid = nullptr;
     double2* opmat = nullptr;
     float2* opmat32 = nullptr;     // FP32 copy of the operator table (mixed mode)

@@ -1592,14 +1592,16 @@ __global__ void zero_kernel(V* p, int64_t n) {
         p[e] = mk<V>(0, 0);
 }
 
+// The caller's densities are read in input order (coalesced: random 16-byte reads would
+// fetch a whole line each) and written to their leaf-sorted positions.
 template <class V>
-__global__ void gather_kernel(const IoPtrs* __restrict__ io, V* __restrict__ out, const int* __restrict__ perm,
+__global__ void gather_kernel(const IoPtrs* __restrict__ io, V* __restrict__ out, const int* __restrict__ iperm,
                               int64_t N, int nrhs) {
     const double2* __restrict__ in = io->den;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < N * nrhs; e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t r = e / N, i = e % N;
-        const double2 v = in[r * N + perm[i]];
-        out[e] = mk<V>(v.x, v.y);
+        const int64_t r = e / N, j = e % N;
+        const double2 v = in[e];
+        out[r * N + iperm[j]] = mk<V>(v.x, v.y);
     }
 }
 
@@ -1767,7 +1769,7 @@ hipError_t launch_stage_t(const DevPlan& d, const Stage& s, int nrhs, hipStream_
         break;
     }
     case ST_GATHER:
-        gather_kernel<V><<<grid_for(d.N * nrhs, 256), 256, 0, st>>>(d.io, (V*)d.bt.p[BUF_DEN], d.perm, d.N, nrhs);
+        gather_kernel<V><<<grid_for(d.N * nrhs, 256), 256, 0, st>>>(d.io, (V*)d.bt.p[BUF_DEN], d.iperm, d.N, nrhs);
         break;
     case ST_SCATTER:
         scatter_kernel<V><<<grid_for(d.N * nrhs, 256), 256, 0, st>>>((const V*)d.bt.p[BUF_VAL], d.io, d.perm, d.N,

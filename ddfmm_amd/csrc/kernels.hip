@@ -408,9 +408,15 @@ __device__ __forceinline__ void kmfma_mirror(kacc4 (&acc)[4][NB], const double (
     }
 }
 
-// (the 8-RHS mirror instance fits 168 VGPRs without spills: three waves per SIMD)
+// Waves per SIMD the register allocation targets: the 8-RHS instance at five (96 VGPRs, a
+// few spilled: G on the vector ALU needs more waves in flight than the 111 VGPRs of a
+// free allocation allow; config 3 near field 16.5 -> 15.3 ms), its mirror instance at
+// three (168 VGPRs without spills); the 16-RHS instances are held by LDS to three and two
+// blocks per CU and keep their free allocation.
+template <int NB, bool MIR> struct KmfmaOcc { static constexpr int waves = NB == 1 ? (MIR ? 3 : 5) : 1; };
+
 template <int NB, int KT, bool MIR>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIR && NB == 1 ? 3 : 1))) void kapply_mfma_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KmfmaOcc<NB, MIR>::waves))) void kapply_mfma_kernel(
     const KTask* __restrict__ tasks, const KSrc* __restrict__ srcs,
     const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
     BufTable bt, int task0, int ntask, int nrhs) {

@@ -1273,35 +1273,44 @@ __global__ __launch_bounds__(256) void fft_fwd_kernel(const FftSrc* __restrict__
     }
 }
 
-// Forward transforms of R right-hand sides of one source per block (nrhs > 1): the x and
-// y passes run per right-hand side into its own slab of B, then the z pass runs over
-// (line, rhs) with the right-hand side fastest, so the spectra, stored right-hand side
-// fastest, are written in runs of R consecutive values.
-template <class V, int R>
+// Forward transforms of R right-hand sides of one source per block (nrhs > 1).  The
+// scatter into the 6^3 sub-grids and the x and y passes run H right-hand sides at a time
+// (H x 36 and H x 72 line transforms per pass over the block's threads) into each one's
+// slab of B; the z pass then runs over (line, rhs) with the right-hand side fastest, so the
+// spectra, stored right-hand side fastest, are written in runs of R consecutive values
+// (64 bytes: whole write sectors).
+template <class V, int R, int H>
 __global__ __launch_bounds__(256) void fft_fwd_multi_kernel(const FftSrc* __restrict__ fs,
                                                             const int* __restrict__ grid, int NL, BufTable bt,
                                                             int f0, int nf, int nrhs) {
-    __shared__ V sub[216], A[432], B[R][864];
+    __shared__ V sub[H][216], A[H][432], B[R][864];
     const int t = threadIdx.x;
     const int r0 = blockIdx.y * R, nr = min(R, nrhs - r0);
+    const int64_t is = bt.stride[BUF_LF_UPEQ];
     for (int bi = blockIdx.x; bi < nf; bi += gridDim.x) {
         const FftSrc f = fs[f0 + xcd_item(bi, nf)];
-        for (int r = 0; r < nr; ++r) {
-            const V* in = bufp<V>(bt, BUF_LF_UPEQ) + (int64_t)(r0 + r) * bt.stride[BUF_LF_UPEQ] + f.in_off;
+        for (int h0 = 0; h0 < nr; h0 += H) {
+            const int hn = min(H, nr - h0);
+            const V* in = bufp<V>(bt, BUF_LF_UPEQ) + (int64_t)(r0 + h0) * is + f.in_off;
             __syncthreads();
-            if (t < 216) sub[t] = mk<V>(0, 0);
+            for (int e = t; e < H * 216; e += blockDim.x) sub[e / 216][e % 216] = mk<V>(0, 0);
             __syncthreads();
-            for (int k = t; k < NL; k += blockDim.x) {
+            for (int e = t; e < hn * NL; e += blockDim.x) {
+                const int r = e / NL, k = e % NL;
                 const int g = grid[f.grid + k];
-                sub[(g % 12 - 6) + 6 * ((g / 12) % 12 - 6) + 36 * (g / 144 - 6)] = in[k];
+                sub[r][(g % 12 - 6) + 6 * ((g / 12) % 12 - 6) + 36 * (g / 144 - 6)] = in[(int64_t)r * is + k];
             }
             __syncthreads();
-            if (t < 36)      // x lines (y', z'): 6 inputs -> 12 outputs, A[z'][y'][kx]
-                dft12_line<V, 6, 12, 6, 0, -1>(sub + 6 * t, 1, A + 12 * t, 1);
+            for (int e = t; e < hn * 36; e += blockDim.x) {    // x lines (y', z'): 6 inputs -> 12 outputs
+                const int r = e / 36, l = e % 36;
+                dft12_line<V, 6, 12, 6, 0, -1>(sub[r] + 6 * l, 1, A[r] + 12 * l, 1);
+            }
             __syncthreads();
-            if (t < 72)      // y lines (kx, z'): A -> B[r][z'][ky][kx]
-                dft12_line<V, 6, 12, 6, 0, -1>(A + (t % 12) + 72 * (t / 12), 12, B[r] + (t % 12) + 144 * (t / 12),
-                                               12);
+            for (int e = t; e < hn * 72; e += blockDim.x) {    // y lines (kx, z'): A -> B[rhs][z'][ky][kx]
+                const int r = e / 72, l = e % 72;
+                dft12_line<V, 6, 12, 6, 0, -1>(A[r] + (l % 12) + 72 * (l / 12), 12,
+                                               B[h0 + r] + (l % 12) + 144 * (l / 12), 12);
+            }
         }
         __syncthreads();
         V* out = bufp<V>(bt, BUF_SPEC) + f.spec_off * nrhs + r0;
@@ -1309,7 +1318,6 @@ __global__ __launch_bounds__(256) void fft_fwd_multi_kernel(const FftSrc* __rest
             const int line = q / nr, r = q % nr;
             dft12_line<V, 6, 12, 6, 0, -1>(B[r] + line, 144, out + (int64_t)line * nrhs + r, 144 * nrhs);
         }
-        __syncthreads();
     }
 }
 
@@ -1831,9 +1839,9 @@ hipError_t launch_stage_t(const DevPlan& d, const Stage& s, int nrhs, hipStream_
         if (nrhs == 1) {
             fft_fwd_kernel<V><<<dim3(s.stream ? std::min(n, AUX_BLOCKS) : n, 1), 256, 0, st>>>(d.fsrc, d.grid, d.NL,
                                                                                            d.bt, s.begin, n, 1);
-        } else {    // 8 (FP32) or 4 (FP64) right-hand sides per block: 55 KB of LDS for B
+        } else {    // 8 (FP32) or 4 (FP64) right-hand sides per block, half at a time: 76 KB of LDS
             constexpr int R = sizeof(V) == 8 ? 8 : 4;
-            fft_fwd_multi_kernel<V, R><<<dim3(n, (nrhs + R - 1) / R), 256, 0, st>>>(d.fsrc, d.grid, d.NL, d.bt,
+            fft_fwd_multi_kernel<V, R, R / 2><<<dim3(n, (nrhs + R - 1) / R), 256, 0, st>>>(d.fsrc, d.grid, d.NL, d.bt,
                                                                                  s.begin, n, nrhs);
         }
         break;

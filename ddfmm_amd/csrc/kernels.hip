@@ -1541,41 +1541,51 @@ __global__ __launch_bounds__(VL_THREADS) void vlist_acc_multi_kernel(const VTarg
     }
 }
 
-// V list, inverse: one block per target, the accumulated spectrum back to the 6^3 sub-cube
-// (pruned inverse DFT, 1/(2P)^3 scaling) and added into the target's check values.
-template <class V>
+// V list, inverse: one block per target and group of R right-hand sides, the accumulated
+// spectra back to the 6^3 sub-cube (pruned inverse DFT, 1/(2P)^3 scaling) and added into
+// the target's check values.  Each pass runs the R right-hand sides' lines together
+// (R x 144 x lines read straight from the spectra, R x 72 y lines, R x 36 z lines over the
+// block's threads).  LDS: R x 1296 values (41 KB for R = 4 in FP32 or 2 in FP64).
+template <class V, int R>
 __global__ __launch_bounds__(256) void vlist_inv_kernel(const VTarget* __restrict__ vt, const int* __restrict__ grid,
-                                                        int NL, BufTable bt, int v0, int nv) {
+                                                        int NL, BufTable bt, int v0, int nv, int nrhs) {
     constexpr int G3 = 1728;
-    __shared__ V g0[G3], A[864], B[432];
-    const int rhs = blockIdx.y;
+    __shared__ V A[R][864], B[R][432];
+    const int r0 = blockIdx.y * R, nr = min(R, nrhs - r0);
     const int t = threadIdx.x;
+    const int64_t is = bt.stride[BUF_TSPEC], os = bt.stride[BUF_LF_CHK];
     for (int bi = blockIdx.x; bi < nv; bi += gridDim.x) {
-    const int i = xcd_item(bi, nv);
-    const VTarget tg = vt[v0 + i];
-    const V* in = bufp<V>(bt, BUF_TSPEC) + (int64_t)rhs * bt.stride[BUF_TSPEC] + (int64_t)i * G3;
-    __syncthreads();
-    for (int e = t; e < G3; e += 256) g0[e] = in[e];
-    __syncthreads();
-    if (t < 144)         // x lines (ky, kz): 12 inputs -> x' in [6, 12), A[kz][ky][x']
-        dft12_line<V, 12, 6, 0, 6, 1>(g0 + 12 * t, 1, A + 6 * t, 1);
-    __syncthreads();
-    if (t < 72)          // y lines (x', kz): A -> B[kz][y'][x']
-        dft12_line<V, 12, 6, 0, 6, 1>(A + (t % 6) + 72 * (t / 6), 6, B + (t % 6) + 36 * (t / 6), 6);
-    __syncthreads();
-    if (t < 36)          // z lines (x', y'): B -> C[z'][y'][x'] (reuses A)
-        dft12_line<V, 12, 6, 0, 6, 1>(B + t, 36, A + t, 36);
-    __syncthreads();
-    V* out = bufp<V>(bt, BUF_LF_CHK) + (int64_t)rhs * bt.stride[BUF_LF_CHK] + tg.out_off;
-    const typename Real<V>::T coef = (typename Real<V>::T)(1.0 / G3);
-    for (int k = t; k < NL; k += blockDim.x) {
-        const int g = grid[tg.grid + k];
-        const V v = A[(g % 12 - 6) + 6 * ((g / 12) % 12 - 6) + 36 * (g / 144 - 6)];
-        V o = out[k];
-        o.x += v.x * coef;
-        o.y += v.y * coef;
-        out[k] = o;
-    }
+        const int i = xcd_item(bi, nv);
+        const VTarget tg = vt[v0 + i];
+        const V* in = bufp<V>(bt, BUF_TSPEC) + (int64_t)r0 * is + (int64_t)i * G3;
+        __syncthreads();
+        for (int e = t; e < nr * 144; e += blockDim.x) {   // x lines (ky, kz): 12 inputs -> x' in [6, 12)
+            const int r = e / 144, l = e % 144;
+            dft12_line<V, 12, 6, 0, 6, 1>(in + (int64_t)r * is + 12 * l, 1, A[r] + 6 * l, 1);
+        }
+        __syncthreads();
+        for (int e = t; e < nr * 72; e += blockDim.x) {    // y lines (x', kz): A -> B[kz][y'][x']
+            const int r = e / 72, l = e % 72;
+            dft12_line<V, 12, 6, 0, 6, 1>(A[r] + (l % 6) + 72 * (l / 6), 6, B[r] + (l % 6) + 36 * (l / 6), 6);
+        }
+        __syncthreads();
+        for (int e = t; e < nr * 36; e += blockDim.x) {    // z lines (x', y'): B -> C[z'][y'][x'] (reuses A)
+            const int r = e / 36, l = e % 36;
+            dft12_line<V, 12, 6, 0, 6, 1>(B[r] + l, 36, A[r] + l, 36);
+        }
+        __syncthreads();
+        V* out = bufp<V>(bt, BUF_LF_CHK) + (int64_t)r0 * os + tg.out_off;
+        const typename Real<V>::T coef = (typename Real<V>::T)(1.0 / G3);
+        for (int e = t; e < nr * NL; e += blockDim.x) {
+            const int r = e / NL, k = e % NL;
+            const int g = grid[tg.grid + k];
+            const V v = A[r][(g % 12 - 6) + 6 * ((g / 12) % 12 - 6) + 36 * (g / 144 - 6)];
+            V* o = out + (int64_t)r * os + k;
+            V w = *o;
+            w.x += v.x * coef;
+            w.y += v.y * coef;
+            *o = w;
+        }
     }
 }
 
@@ -1867,8 +1877,11 @@ hipError_t launch_stage_t(const DevPlan& d, const Stage& s, int nrhs, hipStream_
                     d.vtrg, d.vchunk, d.vsrc, d.vpk, opm, d.bt, s.begin, s.buf, nchunk, nrhs);
             }
         }
-        vlist_inv_kernel<V><<<dim3(s.stream ? std::min(n, AUX_BLOCKS) : n, nrhs), 256, 0, st>>>(d.vtrg, d.grid, d.NL,
-                                                                                                d.bt, s.begin, n);
+        {
+            constexpr int R = sizeof(V) == 8 ? 4 : 2;      // right-hand sides per block
+            vlist_inv_kernel<V, R><<<dim3(s.stream ? std::min(n, AUX_BLOCKS) : n, (nrhs + R - 1) / R), 256, 0, st>>>(
+                d.vtrg, d.grid, d.NL, d.bt, s.begin, n, nrhs);
+        }
         break;
     }
     case ST_REDUCE:

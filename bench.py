@@ -112,16 +112,27 @@ def serve_bench(names, qps, count, slot, precision, seed=4):
     return out
 
 
+def _vlist_name(vt, nrhs):
+    """The V-list accumulation instance the launcher picks for nrhs (kernels.hip)."""
+    if nrhs == 1:
+        return f"vlist_acc_kernel<{vt}>"
+    r2 = 2 if vt == "float2" else 1
+    f, r1 = (1, 8) if nrhs >= 8 * r2 else (2, 4) if nrhs >= 4 * r2 else (4, 2)
+    return f"vlist_acc_multi_kernel<{vt},{f},{r1}>"
+
+
+# kernel family names as the kernel trace prints them (dominant class -> instance)
 KNAME = {
-    "gemm_translation": lambda vt, rc: "gemm_kernel<float2,float2,float>" if vt == "float2"
+    "gemm_translation": lambda vt, n: "gemm_kernel<float2,float2,float>" if vt == "float2"
     else "gemm_kernel<double2,double2,double>",
-    "gemm_solve": lambda vt, rc: "gemm_kernel<double2,float2,double>" if vt == "float2"
+    "gemm_solve": lambda vt, n: "gemm_kernel<double2,float2,double>" if vt == "float2"
     else "gemm_kernel<double2,double2,double>",
-    "kapply": lambda vt, rc: f"kapply_kernel<{vt},{rc}>",
-    "kapply_mfma": lambda nb, kt: f"kapply_mfma_kernel<{nb},{kt}>",
-    "vlist": lambda vt, rc: f"vlist_acc_kernel<{vt},{rc}>",
-    "fft_fwd": lambda vt, rc: f"fft_fwd_kernel<{vt}>",
-    "reduce": lambda vt, rc: f"reduce_kernel<{vt}>",
+    "kapply": lambda vt, n: f"kapply_kernel<{vt},{rhs_block(n)}>",
+    "kapply_mfma": lambda nb, kt: f"kapply_mfma_kernel<{nb},{kt},false>",
+    "vlist": _vlist_name,
+    "fft_fwd": lambda vt, n: f"fft_fwd_kernel<{vt}>" if n == 1 else
+    f"fft_fwd_multi_kernel<{vt},{8 if vt == 'float2' else 4},{4 if vt == 'float2' else 2}>",
+    "reduce": lambda vt, n: f"reduce_kernel<{vt}>",
 }
 
 
@@ -243,7 +254,7 @@ def stage_detail(w, st, dd, vd, nrhs, precision, workload, kernel=0):
     split = precision == "mixed_split"
     esz = 8 if mixed else 16
     vt = "float2" if mixed else "double2"
-    kname = KNAME.get(dom, lambda vt, rc: dom)(vt, rhs_block(nrhs))
+    kname = KNAME.get(dom, lambda vt, n: dom)(vt, nrhs)
     kmfma = mixed and nrhs >= 4         # contexts of >= 4 RHS apply kernels on the matrix cores
     if dom == "kapply" and kmfma:
         kname = KNAME["kapply_mfma"](2 if nrhs > 8 else 1, kernel)

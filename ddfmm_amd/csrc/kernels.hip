@@ -1849,9 +1849,10 @@ hipError_t launch_stage_t(const DevPlan& d, const Stage& s, int nrhs, hipStream_
         if (nrhs == 1) {
             fft_fwd_kernel<V><<<dim3(s.stream ? std::min(n, AUX_BLOCKS) : n, 1), 256, 0, st>>>(d.fsrc, d.grid, d.NL,
                                                                                            d.bt, s.begin, n, 1);
-        } else {    // 8 (FP32) or 4 (FP64) right-hand sides per block, half at a time: 76 KB of LDS
+        } else {    // 8 (FP32) or 4 (FP64) right-hand sides per block, two at a time: 66 or 76 KB
+            // of LDS (an FP32 block still fits beside two translation-GEMM blocks)
             constexpr int R = sizeof(V) == 8 ? 8 : 4;
-            fft_fwd_multi_kernel<V, R, R / 2><<<dim3(n, (nrhs + R - 1) / R), 256, 0, st>>>(d.fsrc, d.grid, d.NL, d.bt,
+            fft_fwd_multi_kernel<V, R, 2><<<dim3(n, (nrhs + R - 1) / R), 256, 0, st>>>(d.fsrc, d.grid, d.NL, d.bt,
                                                                                  s.begin, n, nrhs);
         }
         break;

@@ -131,7 +131,7 @@ KNAME = {
     "kapply_mfma": lambda nb, kt: f"kapply_mfma_kernel<{nb},{kt},false>",
     "vlist": _vlist_name,
     "fft_fwd": lambda vt, n: f"fft_fwd_kernel<{vt}>" if n == 1 else
-    f"fft_fwd_multi_kernel<{vt},{8 if vt == 'float2' else 4},{4 if vt == 'float2' else 2}>",
+    f"fft_fwd_multi_kernel<{vt},{8 if vt == 'float2' else 4},2>",
     "reduce": lambda vt, n: f"reduce_kernel<{vt}>",
 }
 

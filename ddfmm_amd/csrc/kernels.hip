@@ -622,7 +622,7 @@ template <> struct Lds<double> { static constexpr int SAPAD = 2; using C = doubl
 // is loaded while slab s runs on the matrix cores.  Padding stays within 16 rows, 4 k
 // and 16 columns: the MFMAs stop at the last k step of a segment, and a wave whose
 // columns are all empty, or all lack the current K block, skips the segment's MFMAs.
-template <class TA, class V, class R, int MB>
+template <class TA, class V, class R, int MB, bool G3M = false>
 __global__ __launch_bounds__(256) void gemm_kernel(
     const GGroup* __restrict__ groups, const GTile* __restrict__ tiles,
     const int64_t* __restrict__ colB, const int64_t* __restrict__ colC, const int64_t* __restrict__ colA,
@@ -775,13 +775,21 @@ __global__ __launch_bounds__(256) void gemm_kernel(
         }
     };
 
-    typename M::acc4 cr[MB], ci[MB];
+    // G3M (FP32 contexts of the e^{ikr}/r kernel): three real products per complex one,
+    // summed over k apart: k1 = (Ar + Ai) Br, k2 = Ar (Bi - Br), k3 = Ai (Br + Bi); Re C =
+    // k1 - k3 and Im C = k1 + k2 in the epilogue.  Where |Ar| and |Ai| (or |Br| and |Bi|)
+    // differ widely, k1 and k3 are both about Ai Br and cancel: the e^{ikr} kernel's larger
+    // equivalent densities then lose accuracy, so it keeps four products (Re C in cr, Im C
+    // in ci), as FP64 does.
+    static_assert(!G3M || sizeof(R) == 4, "three products in FP32 only");
+    typename M::acc4 cr[MB], ci[MB], c3[G3M ? MB : 1];
 #pragma unroll
     for (int a = 0; a < MB; ++a)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             cr[a][v] = 0;
             ci[a][v] = 0;
+            if constexpr (G3M) c3[a][v] = 0;
         }
 
     const int bb = (wave * 16 + li) * SB + lk, aa = lk * SA + li;
@@ -800,14 +808,24 @@ __global__ __launch_bounds__(256) void gemm_kernel(
                 ar[a] = av.x;
                 ai[a] = av.y;
             }
+            if constexpr (G3M) {   // cr = sum k1, ci = sum k2, c3 = sum k3
+                const R bd = bi - br, bs = br + bi;
 #pragma unroll
-            for (int a = 0; a < MB; ++a) cr[a] = M::op(ar[a], br, cr[a]);
+                for (int a = 0; a < MB; ++a) cr[a] = M::op(ar[a] + ai[a], br, cr[a]);
 #pragma unroll
-            for (int a = 0; a < MB; ++a) ci[a] = M::op(ar[a], bi, ci[a]);
+                for (int a = 0; a < MB; ++a) ci[a] = M::op(ar[a], bd, ci[a]);
 #pragma unroll
-            for (int a = 0; a < MB; ++a) cr[a] = M::op(ai[a], mbi, cr[a]);
+                for (int a = 0; a < MB; ++a) c3[a] = M::op(ai[a], bs, c3[a]);
+            } else {
 #pragma unroll
-            for (int a = 0; a < MB; ++a) ci[a] = M::op(ai[a], br, ci[a]);
+                for (int a = 0; a < MB; ++a) cr[a] = M::op(ar[a], br, cr[a]);
+#pragma unroll
+                for (int a = 0; a < MB; ++a) ci[a] = M::op(ar[a], bi, ci[a]);
+#pragma unroll
+                for (int a = 0; a < MB; ++a) cr[a] = M::op(ai[a], mbi, cr[a]);
+#pragma unroll
+                for (int a = 0; a < MB; ++a) ci[a] = M::op(ai[a], br, ci[a]);
+            }
         }
     };
     auto next = [&](int i, int k0, int& ni, int& nk0) {
@@ -877,6 +895,10 @@ __global__ __launch_bounds__(256) void gemm_kernel(
             const int m = tl.m0 + 16 * a + M::row(lk, v);
             if (m >= g.M) continue;
             R re = cr[a][v], im = ci[a][v];
+            if constexpr (G3M) {
+                re = cr[a][v] - c3[a][v];
+                im = cr[a][v] + ci[a][v];
+            }
             if (g.soff >= 0) {
                 const TA s = opm[g.soff + m];
                 const R sr = (R)s.x, si = (R)s.y;
@@ -896,13 +918,13 @@ __global__ __launch_bounds__(256) void gemm_kernel(
     }
 }
 
-template <class TA, class V, class R>
+template <class TA, class V, class R, bool G3M = false>
 void launch_gemm(int mb, int n, int nrhs, hipStream_t st, const DevPlan& d, const TA* opm, int t0) {
     const dim3 grid(n * nrhs, 1);
-#define DDFMM_GEMM_MB(X)                                                                                \
-    case X:                                                                                             \
-        gemm_kernel<TA, V, R, X><<<grid, 256, 0, st>>>(d.groups, d.tiles, d.colB, d.colC, d.colA, opm, \
-                                                       d.bt, t0, nrhs);                                 \
+#define DDFMM_GEMM_MB(X)                                                                                     \
+    case X:                                                                                                  \
+        gemm_kernel<TA, V, R, X, G3M><<<grid, 256, 0, st>>>(d.groups, d.tiles, d.colB, d.colC, d.colA, opm, \
+                                                            d.bt, t0, nrhs);                                 \
         break;
     switch (mb) {
         DDFMM_GEMM_MB(1) DDFMM_GEMM_MB(2) DDFMM_GEMM_MB(3) DDFMM_GEMM_MB(4) DDFMM_GEMM_MB(5)
@@ -1839,6 +1861,8 @@ hipError_t launch_stage_t(const DevPlan& d, const Stage& s, int nrhs, hipStream_
             launch_gemm<double2, float2, double>(s.mb, n, nrhs, st, d, d.opmat, s.begin);
         else if (d.mixed && d.opmatbf)
             launch_gemm_bf16s(s.mb, n, nrhs, st, d, s.begin);
+        else if (d.mixed && d.ktype == 0)    // e^{ikr}/r: three real products per complex one
+            launch_gemm<float2, float2, float, true>(s.mb, n, nrhs, st, d, d.opmat32, s.begin);
         else if (d.mixed)
             launch_gemm<float2, float2, float>(s.mb, n, nrhs, st, d, d.opmat32, s.begin);
         else

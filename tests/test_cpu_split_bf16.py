@@ -9,8 +9,15 @@ while the solves, the kernel applications and the FFT stay FP64.
   (what six bf16 MFMAs per product would compute);
 * split-BF16, two terms (x1y1 + x1y2 + x2y1): the cheaper variant.
 
+* FP32 with three real products per complex one (the mixed-mode GEMM of the e^{ikr}/r
+  kernel): k1 = (Ar + Ai) xr, k2 = Ar (xi - xr), k3 = Ai (xr + xi), operand sums and
+  products in FP32, Re = k1 - k3, Im = k1 + k2.
+
 The three-term product keeps the FMM as close to the FP64 FMM as FP32 does (about 5e-8
-on the 5000-point sphere); the two-term product is about 60 times further away.
+on the 5000-point sphere); the two-term product is about 60 times further away.  The
+three-product form stays at the FP32 level for e^{ikr}/r and loses accuracy for e^{ikr}
+(larger equivalent densities: the k1 and k3 terms cancel where |Ar| and |Ai| differ
+widely), which is why that kernel keeps four products.
 """
 import numpy as np
 import pytest
@@ -62,6 +69,23 @@ def fp32_matvec(A: np.ndarray, x: np.ndarray) -> np.ndarray:
     return (np.asarray(A, dtype=np.complex64) @ np.asarray(x, dtype=np.complex64)).astype(np.complex128)
 
 
+def gauss3_matvec(A: np.ndarray, x: np.ndarray) -> np.ndarray:
+    Ar, Ai = np.asarray(A.real, dtype=np.float32), np.asarray(A.imag, dtype=np.float32)
+    xr, xi = np.asarray(x.real, dtype=np.float32), np.asarray(x.imag, dtype=np.float32)
+    k1 = (Ar + Ai).astype(np.float32) @ xr
+    k2 = Ar @ (xi - xr).astype(np.float32)
+    k3 = Ai @ (xr + xi).astype(np.float32)
+    return ((k1 - k3) + 1j * (k1 + k2)).astype(np.complex128)
+
+
+def test_gauss3_product_is_the_complex_product():
+    rng = np.random.default_rng(1)
+    A = (rng.standard_normal((40, 30)) + 1j * rng.standard_normal((40, 30))).astype(np.complex64)
+    x = (rng.standard_normal(30) + 1j * rng.standard_normal(30)).astype(np.complex64)
+    exact = A.astype(np.complex128) @ x.astype(np.complex128)
+    assert np.linalg.norm(gauss3_matvec(A, x) - exact) / np.linalg.norm(exact) < 1e-6
+
+
 def test_bf16_split_is_exact_decomposition():
     rng = np.random.default_rng(0)
     x = rng.standard_normal(10000).astype(np.float32) * np.float32(1e3)
@@ -88,6 +112,7 @@ def problem(ops):
 @pytest.mark.parametrize("name,mv,vs_fmm,vs_direct", [
     ("fp32", fp32_matvec, 1e-6, 1e-5),
     ("bf16x3terms", split_matvec(3), 1e-6, 1e-5),
+    ("fp32x3products", gauss3_matvec, 1e-6, 1e-5),
 ])
 def test_emulated_translations_meet_the_bar(problem, monkeypatch, name, mv, vs_fmm, vs_direct):
     w, den, chk, exact, ref = problem
@@ -107,3 +132,19 @@ def test_two_term_split_loses_accuracy(problem, monkeypatch):
     e2 = R.relerr(w.eval(den), ref)
     print(f"split-bf16 vs FP64 FMM: 3 terms {e3:.2e}, 2 terms {e2:.2e}")
     assert e2 > 30 * e3
+
+
+def test_three_products_lose_accuracy_for_exp_ikr(ops_expr, monkeypatch):
+    """e^{ikr} (5000-point sphere): three products per complex one move the FMM further
+    from the FP64 FMM than four do (1.4 times here; 3 times at config 2 on the GPU)."""
+    K, N = 4.0, 5000
+    pos = geometry.sphere(N, K, 21)
+    den = geometry.densities(N, 21)
+    w = R.Wave3dOracle(pos, K, *ops_expr, ktype=1).setup()
+    ref = w.eval(den)
+    monkeypatch.setattr(R, "translate", fp32_matvec)
+    e4 = R.relerr(w.eval(den), ref)
+    monkeypatch.setattr(R, "translate", gauss3_matvec)
+    e3 = R.relerr(w.eval(den), ref)
+    print(f"e^(ikr) vs FP64 FMM: four products {e4:.2e}, three {e3:.2e}")
+    assert e3 > 1.2 * e4

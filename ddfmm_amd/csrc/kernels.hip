@@ -18,8 +18,8 @@
 //
 // Vectors are double2 (FP64 mode) or float2 (mixed mode).  In the mixed mode the
 // kernel G is evaluated from FP64 coordinate differences with FP32 sqrt/rcp and the
-// hardware sin/cos on the phase reduced to [-1/2, 1/2] revolutions (k = 2*pi, so
-// exp(ikr) = exp(2*pi*i*(r - rint(r)))).
+// hardware sin/cos on the phase reduced to [0, 1) revolutions (k = 2*pi, so
+// exp(ikr) = exp(2*pi*i*fract(r))).
 #include <hip/hip_runtime.h>
 
 #include <mutex>
@@ -70,14 +70,16 @@ __device__ __forceinline__ double2 green64(double dx, double dy, double dz, int 
 }
 
 // FP32 G(r) from FP64 differences: r^2 in FP64 (full-rate FMAs, one conversion), 1/r from
-// v_rsq_f32, r = r^2 / r, and v_sin_f32 / v_cos_f32 on the phase r - rint(r) in
-// revolutions (three transcendental operations).
+// v_rsq_f32, r = r^2 / r, and v_sin_f32 / v_cos_f32 on the phase fract(r) in revolutions
+// (three transcendental operations).
 __device__ __forceinline__ float2 green32(double dx, double dy, double dz, int ktype) {
     float r2 = (float)fma(dz, dz, fma(dy, dy, dx * dx));
     r2 = r2 < 1e-16f ? 1.0f : r2;
     const float ir = __builtin_amdgcn_rsqf(r2);
-    // phase r - rint(r) with the product r2 * ir unrounded
-    const float t = fmaf(r2, ir, -__builtin_rintf(r2 * ir));
+    // phase in [0, 1) revolutions, the argument range of the hardware sin/cos: v_fract_f32
+    // of the rounded r (its rounding, at most 2^-23 revolution for r < 4, is below that of
+    // the rsq approximation in r)
+    const float t = __builtin_amdgcn_fractf(r2 * ir);
     float s = __builtin_amdgcn_sinf(t), c = __builtin_amdgcn_cosf(t);
     if (ktype == 0) return make_float2(c * ir, s * ir);
     return make_float2(c, s);

@@ -1355,10 +1355,10 @@ __global__ __launch_bounds__(256) void fft_fwd_multi_kernel(const FftSrc* __rest
 // slices it stages come from its own L2.  Threads: F frequencies x 256 / F targets.
 template <class V> struct VSlice {
     static constexpr int F = 64 / (int)sizeof(V);          // frequencies per slice
-    // LDS row stride, unpadded: a 16-lane read group is two targets' rows of 16 words each,
-    // which then sit in aligned bank halves and collide only when both are in one half (a
-    // 72-byte pitch shifts each row by an even word count and two rows' windows nearly
-    // always overlap: the 1-RHS accumulation 12% slower)
+    // LDS row stride, unpadded: the two targets' 64-byte rows that a 16-lane read group
+    // takes then sit in aligned bank halves (with a 72-byte pitch their bank windows nearly
+    // always overlap), and the 1-RHS block's LDS (51 KB instead of 56) lets three blocks
+    // share a CU instead of two; together 12% faster than the padded pitch
     static constexpr int SE = F;
 };
 
